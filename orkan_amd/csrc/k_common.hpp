@@ -1,0 +1,93 @@
+// k_common.hpp -- shared device helpers of the gfx950 kernels: complex128
+// arithmetic, storage conversion (complex128 or complex64 in HBM, complex128
+// in registers), index enumeration and layout offsets.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ork_kernels.h"
+
+typedef uint64_t u64;
+typedef double2 cd;   // complex128 in registers: x = re, y = im
+
+#define ORK_SQRT1_2 0.70710678118654752440
+
+__device__ __forceinline__ cd cmk(double re, double im) { cd v; v.x = re; v.y = im; return v; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ cd cneg(cd a) { return cmk(-a.x, -a.y); }
+// multiply by i^k, k in 0..3 (exact)
+__device__ __forceinline__ cd cipow(cd a, int k) {
+    switch (k & 3) {
+    case 0: return a;
+    case 1: return cmk(-a.y, a.x);
+    case 2: return cmk(-a.x, -a.y);
+    default: return cmk(a.y, -a.x);
+    }
+}
+// acc += a * b
+__device__ __forceinline__ void cfma(cd &acc, cd a, cd b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+
+// ---- storage element types ---------------------------------------------------------------
+template <typename S> struct Store;
+template <> struct Store<double2> {
+    static __device__ __forceinline__ cd ld(const double2 *p) { return *p; }
+    static __device__ __forceinline__ void st(double2 *p, cd v) { *p = v; }
+};
+template <> struct Store<float2> {
+    static __device__ __forceinline__ cd ld(const float2 *p) {
+        float2 f = *p;
+        return cmk((double)f.x, (double)f.y);
+    }
+    static __device__ __forceinline__ void st(float2 *p, cd v) {
+        float2 f;
+        f.x = (float)v.x;
+        f.y = (float)v.y;
+        *p = f;
+    }
+};
+template <typename S> __device__ __forceinline__ cd ldc(const S *p) { return Store<S>::ld(p); }
+template <typename S> __device__ __forceinline__ void stc(S *p, cd v) { Store<S>::st(p, v); }
+
+// ---- index enumeration -------------------------------------------------------------------
+// insert a 0 bit at position `pos`
+__device__ __forceinline__ u64 ins0(u64 v, int pos) {
+    const u64 low = v & ((((u64)1) << pos) - 1);
+    return low | ((v ^ low) << 1);
+}
+// insert 0 bits at the ascending positions p[0..k-1]
+__device__ __forceinline__ u64 ins0k(u64 v, const int *p, int k) {
+    for (int j = 0; j < k; ++j) v = ins0(v, p[j]);
+    return v;
+}
+
+// ---- layout offsets ----------------------------------------------------------------------
+// packed lower triangle, column-major, element (r, c) with r >= c
+__device__ __forceinline__ u64 packed_off(u64 dim, u64 r, u64 c) {
+    return c * (2 * dim - c + 1) / 2 + (r - c);
+}
+// tiled, element (r, c) whose tile (r >> L, c >> L) is a stored (lower or diagonal) tile
+__device__ __forceinline__ u64 tiled_off(int L, u64 r, u64 c) {
+    const u64 tr = r >> L, tc = c >> L, m = (((u64)1) << L) - 1;
+    return ((tr * (tr + 1) / 2 + tc) << (2 * L)) + ((r & m) << L) + (c & m);
+}
+
+// triangular decode: w -> (R, C) with R >= C, w = R (R + 1) / 2 + C
+__device__ __forceinline__ void tri_decode(u64 w, u64 &R, u64 &C) {
+    u64 r = (u64)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > w) --r;
+    while ((r + 1) * (r + 2) / 2 <= w) ++r;
+    R = r;
+    C = w - r * (r + 1) / 2;
+}
+
+#define ORK_LAUNCH_CHECK() \
+    do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return (int)e_; } while (0)

@@ -1,0 +1,366 @@
+// k_sv.hip -- statevector kernels: single-qubit pair updates, multi-qubit
+// permutation-with-phase updates, the exp_diag phase sweep, and the
+// diagonal-observable reductions (mean, matel_diag).
+//
+// Formulas per gate follow the reference's pair operations
+// (src/gate/gate_pure.c:66-267) so that results agree to the last bits; the
+// memory traffic is the HBM stream every gate needs: the touched amplitudes are
+// read once and written once, 16 B per complex128 amplitude.
+#include "k_common.hpp"
+
+namespace {
+
+constexpr int SV_THREADS = 256;
+constexpr int SV_UNROLL = 2;      // pairs per thread: two independent 32 B loads in flight
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 2048;  // first-stage partial sums (fixed: deterministic order)
+
+struct Coef {
+    double v[8];
+};
+
+template <int OP>
+__device__ __forceinline__ void pair_op(cd &a, cd &b, const Coef &k) {
+    if constexpr (OP == ORK_SV_MAT) {
+        const cd m00 = cmk(k.v[0], k.v[1]), m01 = cmk(k.v[2], k.v[3]);
+        const cd m10 = cmk(k.v[4], k.v[5]), m11 = cmk(k.v[6], k.v[7]);
+        cd na = cmul(m00, a), nb = cmul(m10, a);
+        cfma(na, m01, b);
+        cfma(nb, m11, b);
+        a = na; b = nb;
+    } else if constexpr (OP == ORK_SV_H) {
+        const cd sm = cadd(a, b), df = csub(a, b);
+        a = cscale(sm, ORK_SQRT1_2); b = cscale(df, ORK_SQRT1_2);
+    } else if constexpr (OP == ORK_SV_HY) {
+        const cd df = csub(a, b), sm = cadd(a, b);
+        a = cscale(df, ORK_SQRT1_2); b = cscale(sm, ORK_SQRT1_2);
+    } else if constexpr (OP == ORK_SV_X) {
+        const cd t = a; a = b; b = t;
+    } else if constexpr (OP == ORK_SV_Y) {
+        const cd na = cmk(b.y, -b.x), nb = cmk(-a.y, a.x);
+        a = na; b = nb;
+    } else if constexpr (OP == ORK_SV_RX) {
+        const double c = k.v[0], s = k.v[1];
+        const cd na = cmk(c * a.x + s * b.y, c * a.y - s * b.x);
+        const cd nb = cmk(s * a.y + c * b.x, -s * a.x + c * b.y);
+        a = na; b = nb;
+    } else if constexpr (OP == ORK_SV_RY) {
+        const double c = k.v[0], s = k.v[1];
+        const cd na = cmk(c * a.x - s * b.x, c * a.y - s * b.y);
+        const cd nb = cmk(s * a.x + c * b.x, s * a.y + c * b.y);
+        a = na; b = nb;
+    } else if constexpr (OP == ORK_SV_RZ) {
+        const double c = k.v[0], s = k.v[1];
+        const cd na = cmk(c * a.x + s * a.y, c * a.y - s * a.x);
+        const cd nb = cmk(c * b.x - s * b.y, c * b.y + s * b.x);
+        a = na; b = nb;
+    }
+}
+
+// ops that change only the |1> amplitude of the pair
+template <int OP>
+__device__ __forceinline__ cd one_op(cd b, const Coef &k) {
+    if constexpr (OP == ORK_SV_P) {
+        const double c = k.v[0], s = k.v[1];
+        return cmk(c * b.x - s * b.y, s * b.x + c * b.y);
+    } else if constexpr (OP == ORK_SV_Z) {
+        return cneg(b);
+    } else if constexpr (OP == ORK_SV_S) {
+        return cmk(-b.y, b.x);
+    } else if constexpr (OP == ORK_SV_SDG) {
+        return cmk(b.y, -b.x);
+    } else if constexpr (OP == ORK_SV_T) {
+        return cmk((b.x - b.y) * ORK_SQRT1_2, (b.x + b.y) * ORK_SQRT1_2);
+    } else {
+        return cmk((b.x + b.y) * ORK_SQRT1_2, (b.y - b.x) * ORK_SQRT1_2);
+    }
+}
+
+template <typename S, int OP>
+__global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
+    const u64 base = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 stride = (u64)1 << q;
+    cd a[SV_UNROLL], b[SV_UNROLL];
+    u64 i0[SV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 kk = base + (u64)u * SV_THREADS;
+        i0[u] = ins0(kk, q);
+        if (kk < npairs) {
+            a[u] = ldc(psi + i0[u]);
+            b[u] = ldc(psi + i0[u] + stride);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 kk = base + (u64)u * SV_THREADS;
+        if (kk < npairs) {
+            pair_op<OP>(a[u], b[u], k);
+            stc(psi + i0[u], a[u]);
+            stc(psi + i0[u] + stride, b[u]);
+        }
+    }
+}
+
+template <typename S, int OP>
+__global__ __launch_bounds__(SV_THREADS) void sv_one_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
+    const u64 base = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 stride = (u64)1 << q;
+    cd b[SV_UNROLL];
+    u64 i1[SV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 kk = base + (u64)u * SV_THREADS;
+        i1[u] = ins0(kk, q) + stride;
+        if (kk < npairs) b[u] = ldc(psi + i1[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 kk = base + (u64)u * SV_THREADS;
+        if (kk < npairs) stc(psi + i1[u], one_op<OP>(b[u], k));
+    }
+}
+
+// ---- multi-qubit permutation with phases -------------------------------------------------
+struct MonoArgs {
+    int k, nt;
+    int pos[3];       // ascending gate qubits
+    u64 src[8];       // offsets of the touched amplitudes inside a group
+    u64 dst[8];       // where each touched amplitude goes
+    int ph[8];        // times i^ph
+};
+
+template <typename S>
+__global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi, u64 nbase, MonoArgs m) {
+    const u64 b0 = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    cd v[SV_UNROLL][8];
+    u64 base[SV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 b = b0 + (u64)u * SV_THREADS;
+        base[u] = ins0k(b, m.pos, m.k);
+        if (b < nbase)
+            for (int j = 0; j < m.nt; ++j) v[u][j] = ldc(psi + (base[u] | m.src[j]));
+    }
+#pragma unroll
+    for (int u = 0; u < SV_UNROLL; ++u) {
+        const u64 b = b0 + (u64)u * SV_THREADS;
+        if (b < nbase)
+            for (int j = 0; j < m.nt; ++j) stc(psi + (base[u] | m.dst[j]), cipow(v[u][j], m.ph[j]));
+    }
+}
+
+// ---- exp_diag: psi_x *= exp(-i diag[x] t) ------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(SV_THREADS) void sv_exp_diag_kernel(S *__restrict__ psi, const double *__restrict__ diag,
+                                                                 u64 dim, double t) {
+    const u64 x = (u64)blockIdx.x * SV_THREADS + threadIdx.x;
+    if (x >= dim) return;
+    double s, c;
+    sincos(diag[x] * t, &s, &c);
+    const cd v = ldc(psi + x);
+    stc(psi + x, cmk(c * v.x + s * v.y, c * v.y - s * v.x));
+}
+
+// ---- reductions ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block-wide sum of NV values; result valid in thread 0
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[NV][RED_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        v[i] = wave_sum(v[i]);
+        if (lane == 0) red[i][wv] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int w = 0; w < RED_THREADS / 64; ++w) s += red[i][w];
+            v[i] = s;
+        }
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restrict__ psi, const double *__restrict__ obs,
+                                                              u64 dim, double *__restrict__ part) {
+    double acc[1] = {0.0};
+    for (u64 x = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x < dim; x += (u64)gridDim.x * RED_THREADS) {
+        const cd a = ldc(psi + x);
+        acc[0] += obs[x] * (a.x * a.x + a.y * a.y);
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restrict__ bra, const S *__restrict__ ket,
+                                                               const double *__restrict__ obs, u64 dim,
+                                                               double *__restrict__ part) {
+    double acc[2] = {0.0, 0.0};
+    for (u64 x = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x < dim; x += (u64)gridDim.x * RED_THREADS) {
+        const cd a = ldc(bra + x), b = ldc(ket + x);
+        const double o = obs[x];
+        acc[0] += o * (a.x * b.x + a.y * b.y);
+        acc[1] += o * (a.x * b.y - a.y * b.x);
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = acc[0];
+        part[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+// second stage: nv interleaved sums over nblk partials, written after the partials
+template <int NV>
+__global__ __launch_bounds__(RED_THREADS) void finish_kernel(double *__restrict__ part, int nblk) {
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RED_THREADS)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += part[NV * b + i];
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) part[NV * RED_BLOCKS + i] = acc[i];
+}
+
+template <typename S, int OP>
+void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
+    const u64 per = (u64)SV_THREADS * SV_UNROLL;
+    const unsigned grid = (unsigned)((npairs + per - 1) / per);
+    if (OP == ORK_SV_P || OP == ORK_SV_Z || OP == ORK_SV_S || OP == ORK_SV_SDG || OP == ORK_SV_T ||
+        OP == ORK_SV_TDG)
+        sv_one_kernel<S, OP><<<grid, SV_THREADS, 0, st>>>((S *)psi, npairs, q, k);
+    else
+        sv_pair_kernel<S, OP><<<grid, SV_THREADS, 0, st>>>((S *)psi, npairs, q, k);
+}
+
+template <typename S>
+int sv_1q(hipStream_t st, void *psi, int n, int q, int op, const Coef &k) {
+    const u64 npairs = ((u64)1 << n) >> 1;
+    switch (op) {
+    case ORK_SV_MAT: launch_pair<S, ORK_SV_MAT>(st, psi, npairs, q, k); break;
+    case ORK_SV_H: launch_pair<S, ORK_SV_H>(st, psi, npairs, q, k); break;
+    case ORK_SV_HY: launch_pair<S, ORK_SV_HY>(st, psi, npairs, q, k); break;
+    case ORK_SV_X: launch_pair<S, ORK_SV_X>(st, psi, npairs, q, k); break;
+    case ORK_SV_Y: launch_pair<S, ORK_SV_Y>(st, psi, npairs, q, k); break;
+    case ORK_SV_RX: launch_pair<S, ORK_SV_RX>(st, psi, npairs, q, k); break;
+    case ORK_SV_RY: launch_pair<S, ORK_SV_RY>(st, psi, npairs, q, k); break;
+    case ORK_SV_RZ: launch_pair<S, ORK_SV_RZ>(st, psi, npairs, q, k); break;
+    case ORK_SV_P: launch_pair<S, ORK_SV_P>(st, psi, npairs, q, k); break;
+    case ORK_SV_Z: launch_pair<S, ORK_SV_Z>(st, psi, npairs, q, k); break;
+    case ORK_SV_S: launch_pair<S, ORK_SV_S>(st, psi, npairs, q, k); break;
+    case ORK_SV_SDG: launch_pair<S, ORK_SV_SDG>(st, psi, npairs, q, k); break;
+    case ORK_SV_T: launch_pair<S, ORK_SV_T>(st, psi, npairs, q, k); break;
+    case ORK_SV_TDG: launch_pair<S, ORK_SV_TDG>(st, psi, npairs, q, k); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+// ---- C ABI --------------------------------------------------------------------------------
+extern "C" uint64_t ork_partials_len(void) { return 2 * RED_BLOCKS + 2; }
+
+extern "C" int ork_sv_1q(void *stream, int prec, void *psi, int n, int q, int op, const double *coef) {
+    Coef k;
+    for (int i = 0; i < 8; ++i) k.v[i] = coef ? coef[i] : 0.0;
+    hipStream_t st = (hipStream_t)stream;
+    return prec == ORK_F32 ? sv_1q<float2>(st, psi, n, q, op, k) : sv_1q<double2>(st, psi, n, q, op, k);
+}
+
+extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_mono_t *mono) {
+    MonoArgs m;
+    m.k = mono->k;
+    int srt[3] = {mono->qs[0], mono->qs[1], mono->qs[2]};
+    for (int i = 0; i < m.k; ++i)
+        for (int j = i + 1; j < m.k; ++j)
+            if (srt[j] < srt[i]) { int t = srt[i]; srt[i] = srt[j]; srt[j] = t; }
+    for (int i = 0; i < 3; ++i) m.pos[i] = i < m.k ? srt[i] : 0;
+    auto spread = [&](int a) {
+        u64 v = 0;
+        for (int j = 0; j < mono->k; ++j)
+            if (a >> j & 1) v |= (u64)1 << mono->qs[j];
+        return v;
+    };
+    m.nt = 0;
+    for (int a = 0; a < (1 << m.k); ++a) {
+        if (mono->perm[a] == a && (mono->ph[a] & 3) == 0) continue;
+        m.src[m.nt] = spread(a);
+        m.dst[m.nt] = spread(mono->perm[a]);
+        m.ph[m.nt] = mono->ph[a] & 3;
+        ++m.nt;
+    }
+    if (m.nt == 0) return 0;
+    const u64 nbase = ((u64)1 << n) >> m.k;
+    const u64 per = (u64)SV_THREADS * SV_UNROLL;
+    const unsigned grid = (unsigned)((nbase + per - 1) / per);
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == ORK_F32) sv_mono_kernel<float2><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, nbase, m);
+    else sv_mono_kernel<double2><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, nbase, m);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_sv_exp_diag(void *stream, int prec, void *psi, int n, const double *diag, double t) {
+    const u64 dim = (u64)1 << n;
+    const unsigned grid = (unsigned)((dim + SV_THREADS - 1) / SV_THREADS);
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == ORK_F32) sv_exp_diag_kernel<float2><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, diag, dim, t);
+    else sv_exp_diag_kernel<double2><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, diag, dim, t);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+static int red_blocks(u64 dim) {
+    u64 b = (dim + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
+    if (b < 1) b = 1;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    return (int)b;
+}
+
+extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const double *obs, double *part,
+                           double *out) {
+    const u64 dim = (u64)1 << n;
+    const int nb = red_blocks(dim);
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == ORK_F32) sv_mean_kernel<float2><<<nb, RED_THREADS, 0, st>>>((const float2 *)psi, obs, dim, part);
+    else sv_mean_kernel<double2><<<nb, RED_THREADS, 0, st>>>((const double2 *)psi, obs, dim, part);
+    ORK_LAUNCH_CHECK();
+    finish_kernel<1><<<1, RED_THREADS, 0, st>>>(part, nb);
+    ORK_LAUNCH_CHECK();
+    hipError_t e = hipMemcpyAsync(out, part + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipStreamSynchronize(st);
+}
+
+extern "C" int ork_sv_matel(void *stream, int prec, const void *bra, const void *ket, int n, const double *obs,
+                            double *part, double *out) {
+    const u64 dim = (u64)1 << n;
+    const int nb = red_blocks(dim);
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == ORK_F32)
+        sv_matel_kernel<float2><<<nb, RED_THREADS, 0, st>>>((const float2 *)bra, (const float2 *)ket, obs, dim, part);
+    else
+        sv_matel_kernel<double2><<<nb, RED_THREADS, 0, st>>>((const double2 *)bra, (const double2 *)ket, obs, dim,
+                                                             part);
+    ORK_LAUNCH_CHECK();
+    finish_kernel<2><<<1, RED_THREADS, 0, st>>>(part, nb);
+    ORK_LAUNCH_CHECK();
+    hipError_t e = hipMemcpyAsync(out, part + 2 * RED_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipStreamSynchronize(st);
+}

@@ -1,0 +1,129 @@
+/*
+ * ork_runtime.c -- per-device runtime of the library: one HIP stream per
+ * device (default priority), the reduction partials, and the staging buffers
+ * that bring caller-owned host arrays (observables, diagonals) into HBM.
+ */
+#include "ork_host.h"
+
+#include <pthread.h>
+#include <string.h>
+#include <hip/hip_runtime_api.h>
+
+#define ORK_MAX_DEV 64
+
+static ork_dev_t g_dev[ORK_MAX_DEV];
+static int g_ready[ORK_MAX_DEV];
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
+
+void ork_hip_fail(const char *where, int err) {
+    fprintf(stderr, "orkan: hip: %s: %s\n", where, hipGetErrorString((hipError_t)err));
+    exit(EXIT_FAILURE);
+}
+
+void ork_lock(void) { pthread_mutex_lock(&g_mu); }
+void ork_unlock(void) { pthread_mutex_unlock(&g_mu); }
+
+ork_dev_t *ork_dev(void) {
+    int dev = 0;
+    ORK_HIP("hipGetDevice", hipGetDevice(&dev));
+    if (dev < 0 || dev >= ORK_MAX_DEV) ork_hip_fail("device index", (int)hipErrorInvalidDevice);
+    if (!g_ready[dev]) {
+        pthread_mutex_lock(&g_mu);
+        if (!g_ready[dev]) {
+            ork_dev_t *d = &g_dev[dev];
+            memset(d, 0, sizeof(*d));
+            hipStream_t st;
+            ORK_HIP("hipStreamCreateWithFlags", hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            d->stream = (void *)st;
+            ORK_HIP("hipMalloc(partials)", hipMalloc((void **)&d->partials, ork_partials_len() * sizeof(double)));
+            g_ready[dev] = 1;
+        }
+        pthread_mutex_unlock(&g_mu);
+    }
+    return &g_dev[dev];
+}
+
+static int is_device_ptr(const void *p) {
+    hipPointerAttribute_t at;
+    memset(&at, 0, sizeof(at));
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return at.type == hipMemoryTypeDevice;
+}
+
+const double *ork_dev_doubles(const double *src, size_t count) {
+    if (is_device_ptr(src)) return src;
+    ork_dev_t *d = ork_dev();
+    const size_t bytes = count * sizeof(double);
+    if (d->stage_bytes < bytes) {
+        if (d->stage) {
+            ORK_HIP("hipStreamSynchronize", hipStreamSynchronize((hipStream_t)d->stream));
+            ORK_HIP("hipFree(stage)", hipFree(d->stage));
+        }
+        d->stage = NULL;
+        d->stage_bytes = 0;
+        ORK_HIP("hipMalloc(stage)", hipMalloc(&d->stage, bytes));
+        d->stage_bytes = bytes;
+    }
+    ORK_HIP("hipMemcpyAsync(stage)", hipMemcpyAsync(d->stage, src, bytes, hipMemcpyHostToDevice, (hipStream_t)d->stream));
+    /* the caller may reuse its array as soon as the API call returns */
+    ORK_HIP("hipStreamSynchronize", hipStreamSynchronize((hipStream_t)d->stream));
+    return (const double *)d->stage;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * extension API: stream access, device arrays, timing
+ * ---------------------------------------------------------------------------------------------- */
+
+void *orkan_stream(void) { return ork_dev()->stream; }
+
+int orkan_sync(void) { return (int)hipStreamSynchronize((hipStream_t)ork_dev()->stream); }
+
+double *orkan_dev_alloc(idx_t count) {
+    void *p = NULL;
+    if (hipMalloc(&p, (size_t)count * sizeof(double)) != hipSuccess) return NULL;
+    return (double *)p;
+}
+
+int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count) {
+    ork_dev_t *d = ork_dev();
+    int e = (int)hipMemcpyAsync(dst_dev, src_host, (size_t)count * sizeof(double), hipMemcpyHostToDevice,
+                                (hipStream_t)d->stream);
+    if (e) return e;
+    return (int)hipStreamSynchronize((hipStream_t)d->stream);
+}
+
+void orkan_dev_free(void *p) {
+    if (!p) return;
+    ork_dev();
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p);
+}
+
+#define ORK_MAX_EVENTS 64
+static hipEvent_t g_ev[ORK_MAX_DEV][ORK_MAX_EVENTS];
+
+int orkan_event_record(int slot) {
+    int dev = 0;
+    if (slot < 0 || slot >= ORK_MAX_EVENTS) return (int)hipErrorInvalidValue;
+    ork_dev_t *d = ork_dev();
+    ORK_HIP("hipGetDevice", hipGetDevice(&dev));
+    if (!g_ev[dev][slot]) {
+        int e = (int)hipEventCreate(&g_ev[dev][slot]);
+        if (e) return e;
+    }
+    return (int)hipEventRecord(g_ev[dev][slot], (hipStream_t)d->stream);
+}
+
+double orkan_event_ms(int a, int b) {
+    int dev = 0;
+    float ms = -1.0f;
+    if (a < 0 || b < 0 || a >= ORK_MAX_EVENTS || b >= ORK_MAX_EVENTS) return -1.0;
+    ORK_HIP("hipGetDevice", hipGetDevice(&dev));
+    if (!g_ev[dev][a] || !g_ev[dev][b]) return -1.0;
+    if (hipEventSynchronize(g_ev[dev][b]) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, g_ev[dev][a], g_ev[dev][b]) != hipSuccess) return -1.0;
+    return (double)ms;
+}
