@@ -1,0 +1,251 @@
+"""Benchmark of the state-update hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sv28|dm14] [--no-cpu]
+
+Default workload (sv28): the 28-qubit complex128 statevector brick circuit --
+from |+>^28, each layer applies rx, ry, rz (seeded angles) on every qubit then
+cx on alternating neighbour pairs; a step is one layer (84 rotations + 13 or 14
+cx), the 40 steps of the default run are the whole 40-layer circuit, followed
+by mean() of a seeded diagonal observable (reported separately).
+
+`value` is whole-job HBM throughput: algorithmic bytes of all gates of the K
+timed steps (each amplitude a gate can change read once and written once,
+16 B per amplitude) divided by the time of those steps, with the state and
+the observable resident in HBM.  `roofline` describes the dominant kernel (the
+single-qubit pair update) from per-launch HIP events recorded on the library's
+stream around every rotation of the timed steps.  `cpu_baseline` times the
+reference library compiled from its own sources (oracle/_ref) on this host's
+cores on a bounded sample: one full brick layer at the same 28-qubit size.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "HBM GB/s (% peak) per gate at 28q SV / 14q DM; batched circuit evals/s"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sv28", choices=["sv28", "dm14"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp32", action="store_true", help="complex64 storage variant")
+    return ap.parse_args()
+
+
+class Events:
+    """HIP events on the library's stream (orkan_event_record / orkan_event_ms)."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.n = 0
+
+    def rec(self):
+        i = self.n
+        self.n += 1
+        rc = self.lib.orkan_event_record(i)
+        assert rc == 0, rc
+        return i
+
+    def ms(self, a, b):
+        return self.lib.orkan_event_ms(a, b)
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (profiles/traffic.json, written from the committed counter summaries)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get(kernel_key, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+# -------------------------------------------------------------------------------------------- sv28
+def run_sv28(args, lib):
+    from orkan_amd import ORKAN_FP32, StateT
+    from orkan_amd import workloads as wl
+
+    n = wl.SV_QUBITS
+    steps = args.steps if args.steps is not None else wl.SV_LAYERS
+    layers, g = wl.brick_circuit(n, max(steps + args.warmup, 1))
+    obs = wl.sv_observable(n, g)
+    elem = 8 if args.fp32 else 16
+    kind = 0 | (ORKAN_FP32 if args.fp32 else 0)
+
+    st = StateT(kind, None, 0)
+    lib.state_plus(C.byref(st), n)
+    assert st.data, "state allocation failed"
+    obs_dev = lib.orkan_dev_alloc(1 << n)
+    assert obs_dev
+    assert lib.orkan_dev_upload(obs_dev, obs.ctypes.data, 1 << n) == 0
+    ps = C.byref(st)
+    fns = {name: getattr(lib, name) for name in ("rx", "ry", "rz", "cx")}
+
+    def apply(op):
+        if op[0] == "cx":
+            fns["cx"](ps, op[1], op[2])
+        else:
+            fns[op[0]](ps, op[1], op[2])
+
+    log(f"state and observable resident ({n} qubits); warmup {args.warmup} layers")
+    for l in range(args.warmup):
+        for op in layers[l]:
+            apply(op)
+    lib.orkan_sync()
+    log("timed layers start")
+
+    ev = Events(lib)
+    rot_pairs, cx_pairs, step_marks = [], [], []
+    total_bytes = 0
+    step_marks.append(ev.rec())
+    t0 = time.perf_counter()
+    for l in range(args.warmup, args.warmup + steps):
+        for op in layers[l]:
+            a = ev.rec()
+            apply(op)
+            b = ev.rec()
+            (cx_pairs if op[0] == "cx" else rot_pairs).append((a, b, op[0]))
+            total_bytes += wl.sv_gate_bytes(op[0], n, elem)
+        step_marks.append(ev.rec())
+    lib.orkan_sync()
+    wall = time.perf_counter() - t0
+    log(f"timed layers done in {wall:.2f} s")
+    total_ms = ev.ms(step_marks[0], step_marks[-1])
+    step_ms = [ev.ms(step_marks[i], step_marks[i + 1]) for i in range(len(step_marks) - 1)]
+    rot_ms = np.array([ev.ms(a, b) for a, b, _ in rot_pairs])
+    rot_names = np.array([nm for _, _, nm in rot_pairs])
+    cx_ms = np.array([ev.ms(a, b) for a, b, _ in cx_pairs])
+
+    # mean() once, timed on its own (reduction reads psi and obs: 24 B per amplitude)
+    m0 = ev.rec()
+    mval = lib.mean(ps, C.cast(obs_dev, C.POINTER(C.c_double)))
+    m1 = ev.rec()
+    mean_ms = ev.ms(m0, m1)
+
+    rot_bytes = wl.sv_gate_bytes("rx", n, elem)
+    rot_avg = float(rot_ms.mean())
+    achieved = rot_bytes / (rot_avg * 1e-3) / 1e9
+    per_gate = {}
+    for name in ("rx", "ry", "rz"):
+        per_gate[name] = round(rot_bytes / (float(rot_ms[rot_names == name].mean()) * 1e-3) / 1e9, 1)
+    cx_bytes = wl.sv_gate_bytes("cx", n, elem)
+    per_gate["cx"] = round(cx_bytes / (float(cx_ms.mean()) * 1e-3) / 1e9, 1)
+    per_gate["mean"] = round(24 * (1 << n) * (elem / 16) / (mean_ms * 1e-3) / 1e9, 1) if mean_ms > 0 else None
+
+    value = total_bytes / (total_ms * 1e-3) / 1e9
+    lib.orkan_dev_free(obs_dev)
+    lib.state_free(ps)
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if args.fp32 else "c128",
+        "data": "synthetic",
+        "config": {"workload": f"statevector {n} qubits, brick layers of rx/ry/rz on every qubit + cx on "
+                               f"alternating pairs, from |+>^{n}",
+                   "qubits": n, "layers_timed": steps, "gates_per_step": len(layers[0]),
+                   "storage": "complex64" if args.fp32 else "complex128", "seed": hex(wl.SEED_BASE + 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_from_profiles("sv_pair_kernel_rot_28q" + ("_fp32" if args.fp32 else "")),
+                     "kernel": "sv_pair_kernel (rx/ry/rz)", "bytes_per_launch": rot_bytes,
+                     "avg_launch_ms": round(rot_avg, 4), "launches": int(rot_ms.size)},
+        "per_gate_GBs": per_gate,
+        "pct_peak_per_gate": {k: (round(v / PEAK_HBM_GBS * 100, 1) if v else None) for k, v in per_gate.items()},
+        "mean_value": mval,
+        "wall_s": round(wall, 3),
+        "step_ms_minmax": [round(min(step_ms), 3), round(max(step_ms), 3)],
+    }
+    return result, (layers[args.warmup] if steps > 0 else layers[0]), n
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline_sv(layer, n, elem=16, budget_s=20.0):
+    """The reference library (oracle/_ref, compiled from its sources) on this
+    host at the same n, OpenMP over all cores: the gates of one brick layer in
+    order, stopping after `budget_s` seconds of CPU work."""
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    from orkan_amd import _ffi
+    kind = "reference"
+    path = os.path.join(ROOT, "oracle", "_ref", "liborkan_ref_t5.so")
+    if not os.path.exists(path):
+        kind = "port"
+        path = os.path.join(ROOT, "oracle", "lib", "liborkan_oracle_t5.so")
+    if not os.path.exists(path):
+        return None
+    ref = _ffi.bind(path)
+    st = _ffi.StateT(0, None, 0)
+    ref.state_plus(C.byref(st), n)
+    ps = C.byref(st)
+    # one untimed gate to fault the pages in
+    ref.rx(ps, 0, 0.1)
+    t0 = time.perf_counter()
+    nbytes = 0
+    done = 0
+    from orkan_amd import workloads as wl
+    for op in layer:
+        if op[0] == "cx":
+            ref.cx(ps, op[1], op[2])
+        else:
+            getattr(ref, op[0])(ps, op[1], op[2])
+        nbytes += wl.sv_gate_bytes(op[0], n, elem)
+        done += 1
+        if done % 8 == 0:
+            log(f"cpu baseline: {done} gates, {time.perf_counter() - t0:.1f} s")
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    ref.state_free(ps)
+    return {"value": round(nbytes / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": kind,
+            "sample": f"first {done} of the {len(layer)} gates of one brick layer of the same {n}-qubit "
+                      f"circuit, complex128, {dt:.2f} s", "seconds": round(dt, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
+              "note, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+    import orkan_amd
+    lib = orkan_amd.load(5)
+    if args.workload == "sv28":
+        result, sample_layer, n = run_sv28(args, lib)
+        log(f"gpu: {result['value']} GB/s; cpu baseline next")
+        if not args.no_cpu:
+            result["cpu_baseline"] = cpu_baseline_sv(sample_layer, n)
+    else:
+        from orkan_amd import bench_dm
+        result = bench_dm.run(args, lib, Events)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,9 @@ int orkan_state_download(const state_t *state, void *host);
 double *orkan_dev_alloc(idx_t count);
 int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count);
 void orkan_dev_free(void *p);
+/* byte copies between host memory and device memory from orkan_dev_alloc (stream-ordered, then waited for) */
+int orkan_copy_to_host(void *dst_host, const void *src_dev, size_t bytes);
+int orkan_copy_to_device(void *dst_dev, const void *src_host, size_t bytes);
 
 /* ------------------------------------------------------------------------------------------------
  * Stream and timing.  All kernels of the library run in order on one stream
@@ -50,8 +53,62 @@ void orkan_dev_free(void *p);
  * ---------------------------------------------------------------------------------------------- */
 void *orkan_stream(void);                 /* the library's hipStream_t on the current device */
 int orkan_sync(void);                     /* wait for all enqueued work */
-int orkan_event_record(int slot);         /* record event `slot` (0..63) on the stream */
+int orkan_event_record(int slot);         /* record event `slot` (0..65535) on the stream */
 double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded events */
+
+/* ------------------------------------------------------------------------------------------------
+ * Batched QAOA evaluation.  One evaluation is the reference call sequence
+ *   state_plus(PURE, n); p x { exp_diag(C, gamma_l); rx(q, 2 beta_l) for q = 0..n-1 }; mean(C)
+ * for a graph's cost diagonal C (2^n doubles).  All evaluations of one call
+ * run in a single persistent kernel launch replayed from a hipGraph; states of
+ * up to 2^LT amplitudes stay in LDS, larger ones (n <= 2 LT) stream through HBM
+ * in two LDS-tiled sweeps per layer.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct orkan_qaoa orkan_qaoa_t;
+
+/* diags[g]: host or device array of 2^nq[g] doubles, copied at creation (1 <= nq <= 26) */
+orkan_qaoa_t *orkan_qaoa_create(int ngraphs, const unsigned char *nq, const double *const *diags, int max_evals,
+                                int max_p);
+void orkan_qaoa_destroy(orkan_qaoa_t *q);
+/* evaluation e: graph[e], p[e] layers (1..max_p), gamma/beta rows of max_p values
+ * (row e, first p[e] used); out[e] = <C> after the circuit.  0 on success. */
+int orkan_qaoa_eval(orkan_qaoa_t *q, int nevals, const int *graph, const int *p, const double *gamma,
+                    const double *beta, double *out);
+/* launch configuration: LDS tile 2^lt amplitudes (10..13), workgroup size, workgroups per CU (0: max) */
+int orkan_qaoa_set_config(orkan_qaoa_t *q, int lt, int threads, int blocks_per_cu);
+int orkan_qaoa_get_config(const orkan_qaoa_t *q, int *lt, int *threads, int *grid);
+/* time every candidate configuration on this evaluation list and keep the fastest */
+int orkan_qaoa_autotune(orkan_qaoa_t *q, int nevals, const int *graph, const int *p, const double *gamma,
+                        const double *beta, int reps, double *best_ms);
+/* MaxCut cost diagonal C(x) = sum_e w_e [x_ei != x_ej] computed on the device into out_dev (2^n doubles) */
+int orkan_maxcut_diag_dev(int n, int nedges, const int *ei, const int *ej, const double *w, double *out_dev);
+
+/* ------------------------------------------------------------------------------------------------
+ * Energy surrogate: an MLP in_dim -> hidden (x nhidden, GELU) -> 1 trained and
+ * evaluated with BF16 MFMA GEMMs (fp32 accumulation) and Adam on fp32 master
+ * weights.  Inputs are fp32 rows of in_dim features (host or device memory).
+ * in_dim must be a multiple of 32 and hidden of 128.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct orkan_mlp orkan_mlp_t;
+
+orkan_mlp_t *orkan_mlp_create(int in_dim, int hidden, int nhidden, int max_batch);
+void orkan_mlp_destroy(orkan_mlp_t *m);
+/* W[l]: fp32 [K_l][hidden] row-major (K_0 = in_dim, else hidden); b[l]: [hidden]; w_out: [hidden] */
+int orkan_mlp_set_weights(orkan_mlp_t *m, const float *const *W, const float *const *b, const float *w_out,
+                          float b_out);
+int orkan_mlp_get_weights(orkan_mlp_t *m, float *const *W, float *const *b, float *w_out, float *b_out);
+int orkan_mlp_forward(orkan_mlp_t *m, int batch, const float *x, float *y);
+/* screening: predictions for ncand rows [gf (ng floats), angles_dev[r] (in_dim - ng floats)], with
+ * the rows assembled on the device; angles_dev and y_dev are device arrays (ng <= max_batch) */
+int orkan_mlp_forward_candidates(orkan_mlp_t *m, const float *gf_host, int ng, const float *angles_dev, long ncand,
+                                 float *y_dev);
+/* one Adam step on the mean squared error of the batch (batch <= max_batch) */
+int orkan_mlp_train_step(orkan_mlp_t *m, int batch, const float *x, const float *target, float lr, float *loss);
+/* gradients of the mean squared error without updating the weights */
+int orkan_mlp_grads(orkan_mlp_t *m, int batch, const float *x, const float *target, float *const *dW,
+                    float *const *db, float *dw_out, float *db_out, float *loss);
+/* fp32 C [M][N] = A [M][K] B [N][K]^T for bf16 device arrays (M, N multiples of 128, K of 32) */
+int orkan_mlp_gemm_test(int M, int N, int K, const void *A, const void *B, float *C);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,8 @@ EXTRA = {
     "orkan_dev_alloc": (C.c_void_p, [C.c_uint64]),
     "orkan_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "orkan_dev_free": (None, [C.c_void_p]),
+    "orkan_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "orkan_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "orkan_stream": (C.c_void_p, []),
     "orkan_sync": (C.c_int, []),
     "orkan_event_record": (C.c_int, [C.c_int]),
@@ -47,7 +49,9 @@ def load(log_tile=5):
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is missing: build it with `python -m orkan_amd.build`")
         extra = dict(EXTRA)
-        from . import batch as _batch  # noqa: F401  (extra entry points of the batched path)
-        extra.update(getattr(_batch, "EXTRA", {}))
+        from . import batch as _batch
+        from . import surrogate as _sur
+        extra.update(_batch.EXTRA)
+        extra.update(_sur.EXTRA)
         _cache[log_tile] = _ffi.bind(path, extra)
     return _cache[log_tile]

@@ -1,0 +1,112 @@
+"""Density-matrix benchmark workload (14 qubits, both layouts): one step is one
+layer of h and ry on every qubit, cz on alternating neighbours, amplitude
+damping (gamma 0.05) and depolarizing (p 0.02) on every qubit.
+
+Used by bench.py (--workload dm14, and the dm14 block of the default run).
+Per-kernel HBM GB/s use algorithmic bytes on the L stored entries: 32 L for
+mixing gates and channels, 16 L for phase gates, 12 L for cz.
+"""
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+
+from . import workloads as wl
+from ._ffi import StateT, SuperopT
+
+PEAK = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_rho_layout(n, kind, g, rank=16, log_tile=5):
+    """rho = A A^dagger / tr with A a D x rank complex Gaussian, written straight
+    into the storage layout block by block (no dense D x D matrix)."""
+    D = 1 << n
+    A = g.standard_normal((D, rank)) + 1j * g.standard_normal((D, rank))
+    A /= np.sqrt(np.sum(np.abs(A) ** 2))
+    out = np.empty(wl.dm_stored_len(n, kind, log_tile), dtype=np.complex128)
+    if kind == 1:   # packed, column-major lower triangle
+        pos = 0
+        for c in range(D):
+            col = A[c:] @ A[c].conj()
+            out[pos:pos + D - c] = col
+            pos += D - c
+    else:           # tiled, lower tiles row by row
+        td = 1 << log_tile
+        nt = D // td
+        pos = 0
+        for tr in range(nt):
+            blk = A[tr * td:(tr + 1) * td] @ A[:(tr + 1) * td].conj().T      # td x (tr+1) td
+            tiles = blk.reshape(td, tr + 1, td).transpose(1, 0, 2)            # (tc, lr, lc)
+            out[pos:pos + (tr + 1) * td * td] = tiles.reshape(-1)
+            pos += (tr + 1) * td * td
+    return out
+
+
+def run(args, lib, Events, layouts=(2, 1), layers=None, cpu=True):
+    n = wl.DM_QUBITS
+    steps = args.steps if args.steps is not None else wl.DM_LAYERS
+    g = wl.rng(3)
+    thetas = g.uniform(0, 2 * np.pi, size=(steps + args.warmup, n))
+    Sad = np.ascontiguousarray(wl.superop(wl.kraus_amp_damp(0.05)).reshape(-1))
+    Sdp = np.ascontiguousarray(wl.superop(wl.kraus_depolarizing(0.02)).reshape(-1))
+    sad = SuperopT(1, Sad.ctypes.data)
+    sdp = SuperopT(1, Sdp.ctypes.data)
+    res = {}
+    for kind in layouts:
+        name = {1: "packed", 2: "tiled"}[kind]
+        L = wl.dm_stored_len(n, kind)
+        data = build_rho_layout(n, kind, wl.rng(3))
+        st = StateT(kind, None, 0)
+        lib.state_init(C.byref(st), n, None)
+        assert st.data
+        assert lib.orkan_state_upload(C.byref(st), data.ctypes.data) == 0
+        del data
+        ps = C.byref(st)
+
+        def apply(op):
+            if op[0] == "ampdamp":
+                lib.channel_1q(ps, C.byref(sad), op[1])
+            elif op[0] == "depol":
+                lib.channel_1q(ps, C.byref(sdp), op[1])
+            elif op[0] == "cz":
+                lib.cz(ps, op[1], op[2])
+            elif op[0] == "ry":
+                lib.ry(ps, op[1], op[2])
+            else:
+                getattr(lib, op[0])(ps, op[1])
+
+        for l in range(args.warmup):
+            for op in wl.dm_layer(n, l, thetas[l]):
+                apply(op)
+        lib.orkan_sync()
+        log(f"dm14 {name}: timed layers start")
+        ev = Events(lib)
+        per = {}
+        total_bytes = 0
+        m0 = ev.rec()
+        for l in range(args.warmup, args.warmup + steps):
+            for op in wl.dm_layer(n, l, thetas[l]):
+                a = ev.rec()
+                apply(op)
+                b = ev.rec()
+                key = "channel" if op[0] in ("ampdamp", "depol") else op[0]
+                per.setdefault(key, []).append((a, b))
+                total_bytes += wl.dm_gate_bytes(key, L)
+        m1 = ev.rec()
+        lib.orkan_sync()
+        tot = ev.ms(m0, m1)
+        kern = {}
+        for key, pairs in per.items():
+            ms = float(np.mean([ev.ms(a, b) for a, b in pairs]))
+            gbs = wl.dm_gate_bytes(key, L) / (ms * 1e-3) / 1e9
+            kern[key] = {"avg_ms": round(ms, 4), "GB/s": round(gbs, 1), "pct_peak": round(gbs / PEAK * 100, 1)}
+        res[name] = {"stored_entries": L, "GB/s": round(total_bytes / (tot * 1e-3) / 1e9, 1),
+                     "ms_per_layer": round(tot / steps, 3), "kernels": kern}
+        log(f"dm14 {name}: {res[name]['GB/s']} GB/s")
+        lib.state_free(ps)
+    return res

@@ -1,0 +1,309 @@
+// k_mlp.hip -- BF16 MFMA kernels of the QAOA energy surrogate (an MLP
+// 128 -> 512 -> 512 -> 512 -> 1 with GELU): forward, backward and Adam.
+//
+// The GEMM is C[M x N] = A[M x K] * B[N x K]^T with both operands K-contiguous
+// bf16 and fp32 accumulation on v_mfma_f32_32x32x16_bf16 (gfx950).  A 256-thread
+// workgroup computes a 128 x 128 tile (2 x 2 waves, each 64 x 64 = 2 x 2 MFMA
+// tiles) over K steps of 32, staged through double-buffered LDS.  The epilogue
+// is fused: bias + GELU (forward), GELU' (backward), or plain fp32 (weight
+// gradients).  Shapes must be multiples of the tile (M, N of 128, K of 32);
+// the host pads the batch.
+//
+// No reference counterpart: this is the learned screening stage on top of the
+// simulator.
+#include "k_common.hpp"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int KP = BK + 8;            // padded LDS row (bf16 elements)
+constexpr int GT = 256;               // threads per GEMM workgroup
+
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_BWD = 2, EPI_F32_ACC = 3 };
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+    return cdf + x * pdf;
+}
+
+struct EpiArgs {
+    float *c32;          // EPI_F32 / EPI_F32_ACC: fp32 C, ldc
+    int ldc;
+    const float *bias;   // EPI_BIAS_GELU: bias[N]
+    float *pre;          // EPI_BIAS_GELU: pre-activation out (fp32, ldc); EPI_GELU_BWD: pre-activation in
+    bf16 *act;           // EPI_BIAS_GELU: gelu(pre) out (bf16, ldc); EPI_GELU_BWD: dPre out (bf16, ldc)
+};
+
+template <int EPI>
+__global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B,
+                                                     int ldb, int K, EpiArgs ep) {
+    __shared__ __attribute__((aligned(16))) bf16 As[2][BM * KP];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN * KP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long m0 = (long)blockIdx.y * BM, n0 = (long)blockIdx.x * BN;
+    // global -> register staging: each thread moves 16 bf16 of A and 16 of B per K step
+    const int lr = tid >> 1, lk = (tid & 1) * 16;
+    const bf16 *ga = A + (m0 + lr) * (long)lda + lk;
+    const bf16 *gb = B + (n0 + lr) * (long)ldb + lk;
+    uint4 ra[2], rb[2];
+    auto gload = [&](int k0) {
+        ra[0] = *(const uint4 *)(ga + k0);
+        ra[1] = *(const uint4 *)(ga + k0 + 8);
+        rb[0] = *(const uint4 *)(gb + k0);
+        rb[1] = *(const uint4 *)(gb + k0 + 8);
+    };
+    auto sstore = [&](int buf) {
+        *(uint4 *)(&As[buf][lr * KP + lk]) = ra[0];
+        *(uint4 *)(&As[buf][lr * KP + lk + 8]) = ra[1];
+        *(uint4 *)(&Bs[buf][lr * KP + lk]) = rb[0];
+        *(uint4 *)(&Bs[buf][lr * KP + lk + 8]) = rb[1];
+    };
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int KT = K / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int l32 = lane & 31, lhi = lane >> 5;
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) gload((kt + 1) * BK);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int kk = s * 16 + lhi * 8;
+            bf16x8 av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = *(const bf16x8 *)(&As[buf][(wm * 64 + i * 32 + l32) * KP + kk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = *(const bf16x8 *)(&Bs[buf][(wn * 64 + j * 32 + l32) * KP + kk]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < KT) sstore(buf ^ 1);
+        __syncthreads();
+    }
+    // epilogue: lane holds C[row][col] with col = l32, row = 8 (r / 4) + 4 lhi + (r % 4)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
+                const long col = n0 + wn * 64 + j * 32 + l32;
+                const long at = row * ep.ldc + col;
+                const float v = acc[i][j][r];
+                if constexpr (EPI == EPI_F32) ep.c32[at] = v;
+                else if constexpr (EPI == EPI_F32_ACC) ep.c32[at] += v;
+                else if constexpr (EPI == EPI_BIAS_GELU) {
+                    const float z = v + ep.bias[col];
+                    if (ep.pre) ep.pre[at] = z;
+                    ep.act[at] = (bf16)gelu(z);
+                } else {
+                    ep.act[at] = (bf16)(v * gelu_grad(ep.pre[at]));
+                }
+            }
+}
+
+// ---- small kernels -----------------------------------------------------------------------------
+__global__ void transpose_bf16_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int R, int Cc) {
+    __shared__ bf16 t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int y = threadIdx.y; y < 32; y += 8)
+        if (by + y < R && bx + threadIdx.x < Cc) t[y][threadIdx.x] = in[(long)(by + y) * Cc + bx + threadIdx.x];
+    __syncthreads();
+    for (int y = threadIdx.y; y < 32; y += 8)
+        if (bx + y < Cc && by + threadIdx.x < R) out[(long)(bx + y) * R + by + threadIdx.x] = t[threadIdx.x][y];
+}
+
+__global__ void f32_to_bf16_kernel(const float *__restrict__ in, bf16 *__restrict__ out, long n) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) out[i] = (bf16)in[i];
+}
+
+// y[m] = sum_k h[m][k] w[k] + b   (one wave per row); optional loss gradient and
+// per-row squared error (target given)
+__global__ __launch_bounds__(256) void head_forward_kernel(const bf16 *__restrict__ h, const float *__restrict__ w, float b,
+                                                          int M, int K, float *__restrict__ y,
+                                                          const float *__restrict__ target, float *__restrict__ dy,
+                                                          float *__restrict__ sqerr, int valid, float scale) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= M) return;
+    float s = 0.0f;
+    for (int k = lane; k < K; k += 64) s += (float)h[(long)row * K + k] * w[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) {
+        const float v = s + b;
+        y[row] = v;
+        if (target) {
+            const float e = (row < valid) ? (v - target[row]) : 0.0f;
+            dy[row] = 2.0f * e * scale;
+            sqerr[row] = e * e;
+        }
+    }
+}
+
+// dPre3[m][k] = dy[m] w[k] gelu'(pre3[m][k])   (bf16 out)
+__global__ void head_backward_kernel(const float *__restrict__ dy, const float *__restrict__ w,
+                                     const float *__restrict__ pre, bf16 *__restrict__ dpre, int M, int K) {
+    const long n = (long)M * K;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const long m = i / K;
+        const int k = (int)(i % K);
+        dpre[i] = (bf16)(dy[m] * w[k] * gelu_grad(pre[i]));
+    }
+}
+
+// out[k] = sum_m a[m][k] * s[m] (a bf16 [M][K]); bias grads use s = null (sum of rows)
+__global__ __launch_bounds__(256) void colsum_kernel(const bf16 *__restrict__ a, const float *__restrict__ s, int M,
+                                                    int K, float *__restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= K) return;
+    float acc = 0.0f;
+    for (int m = 0; m < M; ++m) acc += (float)a[(long)m * K + k] * (s ? s[m] : 1.0f);
+    out[k] = acc;
+}
+
+__global__ void sum_kernel(const float *__restrict__ v, int n, float *__restrict__ out) {
+    __shared__ float red[4];
+    float acc = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) acc += v[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Adam on fp32 master weights; refresh the bf16 copies W [K][N] and Wt [N][K]
+__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, long n, int ncols, float lr, float b1, float b2, float eps,
+                            float c1, float c2, bf16 *__restrict__ wb, bf16 *__restrict__ wtb, int nrows) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = (bf16)wi;
+        if (wtb) {
+            const long r = i / ncols, c = i % ncols;
+            wtb[c * nrows + r] = (bf16)wi;
+        }
+    }
+}
+
+// X[r] = [gf (ng floats), angles[r0 + r] (na floats)] as fp32 rows of width ng + na
+__global__ void build_features_kernel(float *__restrict__ X, const float *__restrict__ gf, int ng,
+                                      const float *__restrict__ angles, int na, long r0, int rows) {
+    const int width = ng + na;
+    const long n = (long)rows * width;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const long r = i / width;
+        const int c = (int)(i % width);
+        X[i] = c < ng ? gf[c] : angles[(r0 + r) * na + (c - ng)];
+    }
+}
+
+unsigned g1(long n) {
+    long g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------------------------
+// epi: 0 fp32 C, 1 bias+GELU, 2 GELU backward, 3 fp32 C +=
+extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
+                            float *c32, int ldc, const float *bias, float *pre, void *act) {
+    if (M % BM || N % BN || K % BK || M <= 0 || N <= 0 || K <= 0) return (int)hipErrorInvalidValue;
+    EpiArgs ep{c32, ldc, bias, pre, (bf16 *)act};
+    dim3 grid(N / BN, M / BM);
+    hipStream_t st = (hipStream_t)stream;
+    const bf16 *a = (const bf16 *)A, *b = (const bf16 *)B;
+    switch (epi) {
+    case EPI_F32: gemm_nt_kernel<EPI_F32><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+    case EPI_BIAS_GELU: gemm_nt_kernel<EPI_BIAS_GELU><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+    case EPI_GELU_BWD: gemm_nt_kernel<EPI_GELU_BWD><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+    case EPI_F32_ACC: gemm_nt_kernel<EPI_F32_ACC><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_transpose(void *stream, const void *in, void *out, int R, int Cc) {
+    dim3 grid((Cc + 31) / 32, (R + 31) / 32), blk(32, 8);
+    transpose_bf16_kernel<<<grid, blk, 0, (hipStream_t)stream>>>((const bf16 *)in, (bf16 *)out, R, Cc);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_to_bf16(void *stream, const float *in, void *out, long n) {
+    f32_to_bf16_kernel<<<g1(n), 256, 0, (hipStream_t)stream>>>(in, (bf16 *)out, n);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_head_forward(void *stream, const void *h, const float *w, float b, int M, int K, float *y,
+                                    const float *target, float *dy, float *sqerr, int valid, float scale) {
+    head_forward_kernel<<<(M + 3) / 4, 256, 0, (hipStream_t)stream>>>((const bf16 *)h, w, b, M, K, y, target, dy,
+                                                                      sqerr, valid, scale);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_head_backward(void *stream, const float *dy, const float *w, const float *pre, void *dpre,
+                                     int M, int K) {
+    head_backward_kernel<<<g1((long)M * K), 256, 0, (hipStream_t)stream>>>(dy, w, pre, (bf16 *)dpre, M, K);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out) {
+    colsum_kernel<<<(K + 255) / 256, 256, 0, (hipStream_t)stream>>>((const bf16 *)a, s, M, K, out);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_sum(void *stream, const float *v, int n, float *out) {
+    sum_kernel<<<1, 256, 0, (hipStream_t)stream>>>(v, n, out);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_build_features(void *stream, float *X, const float *gf, int ng, const float *angles, int na,
+                                      long r0, int rows) {
+    build_features_kernel<<<g1((long)rows * (ng + na)), 256, 0, (hipStream_t)stream>>>(X, gf, ng, angles, na, r0,
+                                                                                      rows);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_adam(void *stream, float *w, const float *g, float *m, float *v, long n, int ncols, float lr,
+                            float b1, float b2, float eps, float c1, float c2, void *wb, void *wtb, int nrows) {
+    adam_kernel<<<g1(n), 256, 0, (hipStream_t)stream>>>(w, g, m, v, n, ncols, lr, b1, b2, eps, c1, c2, (bf16 *)wb,
+                                                         (bf16 *)wtb, nrows);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}

@@ -1,0 +1,250 @@
+// k_qaoa.hip -- batched QAOA evaluation: thousands of (graph, angles) circuits
+// of uneven qubit counts in one persistent launch.
+//
+// One evaluation is the reference's own call sequence (SURVEY call stack D):
+//   state_plus(PURE, n); p x { exp_diag(C, gamma); rx(q, 2 beta) for q = 0..n-1 };
+//   mean(C)
+// (src/state/state_pure.c:30-43, src/circ/circ_pure.c:21-34,
+//  src/gate/gate_pure.c:221-234, src/meas/meas_pure.c:20-30), with the same
+// per-amplitude formulas and the same qubit order, so each instance matches a
+// per-instance run of those calls to rounding.
+//
+// Execution: every workgroup of the persistent grid pulls evaluations from an
+// atomic counter (largest first) and runs each one start to finish, so no
+// workgroup ever waits for another.  A state of n <= LT qubits lives in LDS for
+// the whole circuit.  A larger state lives in HBM and each layer is two passes
+// of LDS tiles of 2^LT amplitudes: pass A (contiguous tiles) applies the
+// exp_diag phases and the rx of qubits 0..LT-1, pass B (tiles gathered at
+// stride 2^LT) applies the rx of qubits LT..n-1; the last pass also reduces
+// sum_x C(x) |psi_x|^2.  That is 2 sweeps per layer instead of n+1.
+#include "k_common.hpp"
+
+namespace {
+
+struct QaoaEval {
+    int graph;       // index into the diagonal table
+    int p;           // layers
+    int angle_off;   // offset (in doubles) of this evaluation's per-layer (gamma, cos beta, sin beta)
+    int pad;
+};
+
+struct QaoaGraph {
+    u64 diag_off;    // offset (in doubles) of C in the diagonal arena
+    u64 state_off;   // offset (in complex) of the state scratch in the state arena
+    int n;
+    int pad;
+};
+
+struct QaoaArgs {
+    const QaoaEval *evals;
+    const int *order;        // evaluation order (largest work first)
+    const int *nevals;       // device scalar: evaluations in this launch
+    const QaoaGraph *graphs;
+    const double *angles;
+    const double *diag;
+    double2 *states;         // per-workgroup scratch slots for states larger than LDS
+    u64 slot_elems;          // complex elements per scratch slot (2^nmax)
+    double *out;
+    unsigned *counter;
+};
+
+// rx(2 beta) on the pair (a, b): a' = c a - i s b, b' = -i s a + c b (gate_pure.c:221-234)
+__device__ __forceinline__ void rx_pair(cd &a, cd &b, double c, double s) {
+    const cd na = cmk(c * a.x + s * b.y, c * a.y - s * b.x);
+    const cd nb = cmk(s * a.y + c * b.x, -s * a.x + c * b.y);
+    a = na;
+    b = nb;
+}
+
+// psi *= exp(-i C t) (circ_pure.c:27-32)
+__device__ __forceinline__ cd phase(cd v, double d, double t) {
+    double s, c;
+    sincos(d * t, &s, &c);
+    return cmk(c * v.x + s * v.y, c * v.y - s * v.x);
+}
+
+// rx on LDS bits [b0, b1) of a tile of 2^lt amplitudes, qubits in ascending order
+template <int THREADS>
+__device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, double c, double s) {
+    const int npairs = 1 << (lt - 1);
+    for (int b = b0; b < b1; ++b) {
+        for (int k = threadIdx.x; k < npairs; k += THREADS) {
+            const int i0 = (int)ins0((u64)k, b), i1 = i0 | (1 << b);
+            cd a = t[i0], bb = t[i1];
+            rx_pair(a, bb, c, s);
+            t[i0] = a;
+            t[i1] = bb;
+        }
+        __syncthreads();
+    }
+}
+
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < THREADS / 64; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+template <int LT, int THREADS>
+__global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
+    __shared__ cd tile[1 << LT];
+    __shared__ double red[THREADS / 64];
+    __shared__ int item;
+    const int T = 1 << LT;
+    const int nev = *A.nevals;
+    double2 *slot = A.states + (u64)blockIdx.x * A.slot_elems;
+
+    for (;;) {
+        if (threadIdx.x == 0) item = (int)atomicAdd(A.counter, 1u);
+        __syncthreads();
+        const int it = item;
+        __syncthreads();
+        if (it >= nev) break;
+        const int e = A.order[it];
+        const QaoaEval ev = A.evals[e];
+        const QaoaGraph gr = A.graphs[ev.graph];
+        const int n = gr.n;
+        const u64 N = (u64)1 << n;
+        const double *C = A.diag + gr.diag_off;
+        const double *ang = A.angles + ev.angle_off;
+        const double amp = 1.0 / sqrt((double)N);
+        double acc = 0.0;
+
+        if (n <= LT) {
+            // ---- whole state in LDS ----
+            for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[x] = cmk(amp, 0.0);
+            __syncthreads();
+            for (int l = 0; l < ev.p; ++l) {
+                const double g = ang[3 * l], c = ang[3 * l + 1], s = ang[3 * l + 2];
+                for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[x] = phase(tile[x], C[x], g);
+                __syncthreads();
+                lds_rx<THREADS>(tile, n, 0, n, c, s);
+            }
+            for (int x = threadIdx.x; x < (int)N; x += THREADS) {
+                const cd v = tile[x];
+                acc += C[x] * (v.x * v.x + v.y * v.y);
+            }
+        } else {
+            // ---- state in HBM, two LDS-tiled passes per layer ----
+            const int hb = n - LT;                 // qubits above the tile
+            const u64 nchunk = (u64)1 << hb;
+            const int ls = 2 * LT - n;             // log2 of the contiguous run in pass B
+            const int S = 1 << ls;
+            for (int l = 0; l < ev.p; ++l) {
+                const double g = ang[3 * l], c = ang[3 * l + 1], s = ang[3 * l + 2];
+                const bool last = (l == ev.p - 1);
+                // pass A: contiguous tiles, phases + rx on qubits 0..LT-1
+                for (u64 ch = 0; ch < nchunk; ++ch) {
+                    const u64 base = ch << LT;
+                    for (int x = threadIdx.x; x < T; x += THREADS) {
+                        const cd v = (l == 0) ? cmk(amp, 0.0) : slot[base + x];
+                        tile[x] = phase(v, C[base + x], g);
+                    }
+                    __syncthreads();
+                    lds_rx<THREADS>(tile, LT, 0, LT, c, s);
+                    for (int x = threadIdx.x; x < T; x += THREADS) slot[base + x] = tile[x];
+                    __syncthreads();
+                }
+                // pass B: tile = all 2^hb high indices x a run of S low indices
+                for (u64 ch = 0; ch < ((u64)1 << (LT - ls)); ++ch) {
+                    const u64 lo0 = ch << ls;
+                    for (int x = threadIdx.x; x < T; x += THREADS) {
+                        const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
+                        tile[x] = slot[(hi << LT) | lo];
+                    }
+                    __syncthreads();
+                    lds_rx<THREADS>(tile, LT, ls, ls + hb, c, s);
+                    for (int x = threadIdx.x; x < T; x += THREADS) {
+                        const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
+                        const cd v = tile[x];
+                        if (last) acc += C[(hi << LT) | lo] * (v.x * v.x + v.y * v.y);
+                        else slot[(hi << LT) | lo] = v;
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        const double tot = block_sum<THREADS>(acc, red);
+        if (threadIdx.x == 0) A.out[e] = tot;
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter(unsigned *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// C(x) = sum_(i,j) w_ij [x_i != x_j] for one graph
+__global__ __launch_bounds__(256) void maxcut_diag_kernel(double *__restrict__ out, u64 N, int ne,
+                                                         const int *__restrict__ ei, const int *__restrict__ ej,
+                                                         const double *__restrict__ w) {
+    const u64 x = (u64)blockIdx.x * 256 + threadIdx.x;
+    if (x >= N) return;
+    double acc = 0.0;
+    for (int k = 0; k < ne; ++k)
+        if (((x >> ei[k]) ^ (x >> ej[k])) & 1) acc += w[k];
+    out[x] = acc;
+}
+
+template <int LT, int THREADS>
+int launch_qaoa(hipStream_t st, const QaoaArgs &a, int grid) {
+    reset_counter<<<1, 64, 0, st>>>(a.counter);
+    ORK_LAUNCH_CHECK();
+    qaoa_persistent<LT, THREADS><<<grid, THREADS, 0, st>>>(a);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+// ---- C ABI (used by the host module ork_qaoa.c) ----------------------------------------------
+extern "C" int ork_qaoa_launch(void *stream, int lt, int threads, int grid, const void *evals, const int *order,
+                               const int *nevals, const void *graphs, const double *angles, const double *diag,
+                               void *states, uint64_t slot_elems, double *out, unsigned *counter) {
+    QaoaArgs a;
+    a.evals = (const QaoaEval *)evals;
+    a.order = order;
+    a.nevals = nevals;
+    a.graphs = (const QaoaGraph *)graphs;
+    a.angles = angles;
+    a.diag = diag;
+    a.states = (double2 *)states;
+    a.slot_elems = slot_elems;
+    a.out = out;
+    a.counter = counter;
+    hipStream_t st = (hipStream_t)stream;
+#define ORK_QCASE(L, TH) if (lt == L && threads == TH) return launch_qaoa<L, TH>(st, a, grid);
+    ORK_QCASE(10, 256) ORK_QCASE(10, 512) ORK_QCASE(10, 1024)
+    ORK_QCASE(11, 256) ORK_QCASE(11, 512) ORK_QCASE(11, 1024)
+    ORK_QCASE(12, 256) ORK_QCASE(12, 512) ORK_QCASE(12, 1024)
+    ORK_QCASE(13, 512) ORK_QCASE(13, 1024)
+#undef ORK_QCASE
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int ork_qaoa_occupancy(int lt, int threads, int *blocks_per_cu) {
+    hipError_t e = hipErrorInvalidValue;
+#define ORK_QOCC(L, TH) \
+    if (lt == L && threads == TH) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, qaoa_persistent<L, TH>, TH, 0);
+    ORK_QOCC(10, 256) ORK_QOCC(10, 512) ORK_QOCC(10, 1024)
+    ORK_QOCC(11, 256) ORK_QOCC(11, 512) ORK_QOCC(11, 1024)
+    ORK_QOCC(12, 256) ORK_QOCC(12, 512) ORK_QOCC(12, 1024)
+    ORK_QOCC(13, 512) ORK_QOCC(13, 1024)
+#undef ORK_QOCC
+    return (int)e;
+}
+
+extern "C" int ork_maxcut_diag(void *stream, double *out, int n, int ne, const int *ei, const int *ej,
+                               const double *w) {
+    const u64 N = (u64)1 << n;
+    maxcut_diag_kernel<<<(unsigned)((N + 255) / 256), 256, 0, (hipStream_t)stream>>>(out, N, ne, ei, ej, w);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}

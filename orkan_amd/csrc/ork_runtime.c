@@ -95,6 +95,20 @@ int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count) {
     return (int)hipStreamSynchronize((hipStream_t)d->stream);
 }
 
+int orkan_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
+    ork_dev_t *d = ork_dev();
+    int e = (int)hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, (hipStream_t)d->stream);
+    if (e) return e;
+    return (int)hipStreamSynchronize((hipStream_t)d->stream);
+}
+
+int orkan_copy_to_device(void *dst_dev, const void *src_host, size_t bytes) {
+    ork_dev_t *d = ork_dev();
+    int e = (int)hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, (hipStream_t)d->stream);
+    if (e) return e;
+    return (int)hipStreamSynchronize((hipStream_t)d->stream);
+}
+
 void orkan_dev_free(void *p) {
     if (!p) return;
     ork_dev();
@@ -102,14 +116,18 @@ void orkan_dev_free(void *p) {
     (void)hipFree(p);
 }
 
-#define ORK_MAX_EVENTS 64
-static hipEvent_t g_ev[ORK_MAX_DEV][ORK_MAX_EVENTS];
+#define ORK_MAX_EVENTS 65536
+static hipEvent_t *g_ev[ORK_MAX_DEV];
 
 int orkan_event_record(int slot) {
     int dev = 0;
     if (slot < 0 || slot >= ORK_MAX_EVENTS) return (int)hipErrorInvalidValue;
     ork_dev_t *d = ork_dev();
     ORK_HIP("hipGetDevice", hipGetDevice(&dev));
+    if (!g_ev[dev]) {
+        g_ev[dev] = calloc(ORK_MAX_EVENTS, sizeof(hipEvent_t));
+        if (!g_ev[dev]) return (int)hipErrorOutOfMemory;
+    }
     if (!g_ev[dev][slot]) {
         int e = (int)hipEventCreate(&g_ev[dev][slot]);
         if (e) return e;
@@ -122,7 +140,7 @@ double orkan_event_ms(int a, int b) {
     float ms = -1.0f;
     if (a < 0 || b < 0 || a >= ORK_MAX_EVENTS || b >= ORK_MAX_EVENTS) return -1.0;
     ORK_HIP("hipGetDevice", hipGetDevice(&dev));
-    if (!g_ev[dev][a] || !g_ev[dev][b]) return -1.0;
+    if (!g_ev[dev] || !g_ev[dev][a] || !g_ev[dev][b]) return -1.0;
     if (hipEventSynchronize(g_ev[dev][b]) != hipSuccess) return -1.0;
     if (hipEventElapsedTime(&ms, g_ev[dev][a], g_ev[dev][b]) != hipSuccess) return -1.0;
     return (double)ms;
