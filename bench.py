@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--workload", default="sv28", choices=["sv28", "dm14"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32", action="store_true", help="complex64 storage variant")
+    ap.add_argument("--quick", action="store_true", help="statevector line only (no dm14 / qaoa blocks)")
     return ap.parse_args()
 
 
@@ -241,9 +242,14 @@ def main():
         log(f"gpu: {result['value']} GB/s; cpu baseline next")
         if not args.no_cpu:
             result["cpu_baseline"] = cpu_baseline_sv(sample_layer, n)
+        if not args.quick:
+            from orkan_amd import bench_dm, bench_qaoa
+            dm_args = argparse.Namespace(steps=2, warmup=1)
+            result["dm14"] = bench_dm.run(dm_args, lib, Events)
+            result["qaoa"] = bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)
     else:
         from orkan_amd import bench_dm
-        result = bench_dm.run(args, lib, Events)
+        result = {"metric": METRIC, "dm14": bench_dm.run(args, lib, Events)}
     print(json.dumps(result))
 
 

@@ -63,19 +63,41 @@ __device__ __forceinline__ cd phase(cd v, double d, double t) {
     return cmk(c * v.x + s * v.y, c * v.y - s * v.x);
 }
 
-// rx on LDS bits [b0, b1) of a tile of 2^lt amplitudes, qubits in ascending order
+// LDS tiles are padded with one slot after every 8 amplitudes, so that a lane
+// walking 8 consecutive amplitudes and its neighbours hit distinct banks
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
+constexpr int padded(int n) { return n + (n >> 3); }
+
+// rx on LDS bits [b0, b1) of a tile of 2^lt amplitudes, qubits in ascending
+// order, up to three qubits per pass: each thread holds the 8 amplitudes of one
+// 3-bit group in registers and applies the three rx in qubit order, so every
+// amplitude sees the same sequence of operations as gate-by-gate application.
 template <int THREADS>
 __device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, double c, double s) {
-    const int npairs = 1 << (lt - 1);
-    for (int b = b0; b < b1; ++b) {
-        for (int k = threadIdx.x; k < npairs; k += THREADS) {
-            const int i0 = (int)ins0((u64)k, b), i1 = i0 | (1 << b);
-            cd a = t[i0], bb = t[i1];
-            rx_pair(a, bb, c, s);
-            t[i0] = a;
-            t[i1] = bb;
+    for (int b = b0; b < b1;) {
+        const int r = (b1 - b) < 3 ? (b1 - b) : 3;
+        const int R = 1 << r;
+        const int ngroups = 1 << (lt - r);
+        const int lowmask = (1 << b) - 1;
+        for (int gi = threadIdx.x; gi < ngroups; gi += THREADS) {
+            const int base = ((gi & ~lowmask) << r) | (gi & lowmask);
+            cd v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < R) v[j] = t[pad(base | (j << b))];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= r) break;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < R && !((j >> k) & 1)) rx_pair(v[j], v[j | (1 << k)], c, s);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < R) t[pad(base | (j << b))] = v[j];
         }
         __syncthreads();
+        b += r;
     }
 }
 
@@ -94,7 +116,7 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 
 template <int LT, int THREADS>
 __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
-    __shared__ cd tile[1 << LT];
+    __shared__ cd tile[padded(1 << LT)];
     __shared__ double red[THREADS / 64];
     __shared__ int item;
     const int T = 1 << LT;
@@ -119,16 +141,16 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
 
         if (n <= LT) {
             // ---- whole state in LDS ----
-            for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[x] = cmk(amp, 0.0);
+            for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[pad(x)] = cmk(amp, 0.0);
             __syncthreads();
             for (int l = 0; l < ev.p; ++l) {
                 const double g = ang[3 * l], c = ang[3 * l + 1], s = ang[3 * l + 2];
-                for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[x] = phase(tile[x], C[x], g);
+                for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[pad(x)] = phase(tile[pad(x)], C[x], g);
                 __syncthreads();
                 lds_rx<THREADS>(tile, n, 0, n, c, s);
             }
             for (int x = threadIdx.x; x < (int)N; x += THREADS) {
-                const cd v = tile[x];
+                const cd v = tile[pad(x)];
                 acc += C[x] * (v.x * v.x + v.y * v.y);
             }
         } else {
@@ -145,11 +167,11 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
                     const u64 base = ch << LT;
                     for (int x = threadIdx.x; x < T; x += THREADS) {
                         const cd v = (l == 0) ? cmk(amp, 0.0) : slot[base + x];
-                        tile[x] = phase(v, C[base + x], g);
+                        tile[pad(x)] = phase(v, C[base + x], g);
                     }
                     __syncthreads();
                     lds_rx<THREADS>(tile, LT, 0, LT, c, s);
-                    for (int x = threadIdx.x; x < T; x += THREADS) slot[base + x] = tile[x];
+                    for (int x = threadIdx.x; x < T; x += THREADS) slot[base + x] = tile[pad(x)];
                     __syncthreads();
                 }
                 // pass B: tile = all 2^hb high indices x a run of S low indices
@@ -157,13 +179,13 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
                     const u64 lo0 = ch << ls;
                     for (int x = threadIdx.x; x < T; x += THREADS) {
                         const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
-                        tile[x] = slot[(hi << LT) | lo];
+                        tile[pad(x)] = slot[(hi << LT) | lo];
                     }
                     __syncthreads();
                     lds_rx<THREADS>(tile, LT, ls, ls + hb, c, s);
                     for (int x = threadIdx.x; x < T; x += THREADS) {
                         const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
-                        const cd v = tile[x];
+                        const cd v = tile[pad(x)];
                         if (last) acc += C[(hi << LT) | lo] * (v.x * v.x + v.y * v.y);
                         else slot[(hi << LT) | lo] = v;
                     }

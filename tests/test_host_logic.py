@@ -1,0 +1,88 @@
+"""Host-side logic that needs no GPU: workload generators, byte accounting,
+surrogate features, MaxCut diagonals, layout helpers."""
+import numpy as np
+
+from linalg_np import MIXED_PACKED, MIXED_TILED, from_layout, random_rho, state_len, to_layout
+from orkan_amd import batch, surrogate, workloads as wl
+from orkan_amd.bench_dm import build_rho_layout
+
+
+def test_brick_layer_structure():
+    layers, _ = wl.brick_circuit(28, 2)
+    even, odd = layers
+    assert len(even) == 84 + 14 and len(odd) == 84 + 13
+    assert [op[0] for op in even[:3]] == ["rx", "ry", "rz"]
+    assert [op[1:] for op in even if op[0] == "cx"][:2] == [(0, 1), (2, 3)]
+    assert [op[1:] for op in odd if op[0] == "cx"][:2] == [(1, 2), (3, 4)]
+    assert all(0 <= op[2] < 2 * np.pi for op in even if op[0] != "cx")
+
+
+def test_byte_accounting():
+    N = 1 << 28
+    assert wl.sv_gate_bytes("rx", 28) == 8_589_934_592 == 32 * N
+    assert wl.sv_gate_bytes("cx", 28) == 16 * N
+    assert wl.sv_gate_bytes("cz", 28) == 8 * N
+    L = wl.dm_stored_len(14, MIXED_TILED)
+    assert L == 134_479_872 and wl.dm_stored_len(14, MIXED_PACKED) == 134_225_920
+    assert wl.dm_gate_bytes("h", L) == 32 * L and wl.dm_gate_bytes("cz", L) == 12 * L
+    assert wl.dm_gate_bytes("cx", L) == 24 * L and wl.dm_gate_bytes("ccx", L) == 14 * L
+
+
+def test_dm_layer_structure():
+    ops = wl.dm_layer(14, 1, np.zeros(14))
+    kinds = [o[0] for o in ops]
+    assert kinds.count("h") == 14 and kinds.count("ry") == 14 and kinds.count("cz") == 6
+    assert kinds.count("ampdamp") == 14 and kinds.count("depol") == 14
+
+
+def test_low_rank_layout_builder_matches_dense():
+    g = np.random.default_rng(1)
+    for kind in (MIXED_PACKED, MIXED_TILED):
+        n = 7
+        data = build_rho_layout(n, kind, np.random.default_rng(5), rank=4)
+        A = np.random.default_rng(5).standard_normal((1 << n, 4)) + 1j * np.random.default_rng(5).standard_normal((1 << n, 4))
+        full = from_layout(data, kind, n, 5)
+        assert abs(np.trace(full).real - 1.0) < 1e-12
+        assert np.allclose(full, full.conj().T)
+        assert data.size == state_len(kind, n, 5)
+        assert np.linalg.matrix_rank(full, tol=1e-10) == 4
+    del g
+
+
+def test_maxcut_diag_bruteforce():
+    rng = np.random.default_rng(2)
+    i = batch.maxcut_instances(1, rng, n_lo=6, n_hi=6)[0]
+    d = batch.maxcut_diag(6, i["edges"], i["w"])
+    for x in (0, 5, 37, 63):
+        bits = [(x >> k) & 1 for k in range(6)]
+        want = sum(w for (a, b), w in zip(i["edges"], i["w"]) if bits[a] != bits[b])
+        assert abs(d[x] - want) < 1e-12
+    assert d[0] == 0.0 and abs(d.max() - d[::-1].max()) < 1e-12   # C(x) = C(~x)
+
+
+def test_instance_generator_ranges():
+    inst = batch.maxcut_instances(400, wl.rng(4))
+    ns = [i["n"] for i in inst]
+    ps = [i["p"] for i in inst]
+    assert min(ns) == 10 and max(ns) == 20 and min(ps) == 1 and max(ps) == 8
+    for i in inst[:50]:
+        assert i["edges"].shape[0] == i["w"].size
+        assert np.all(i["edges"][:, 0] < i["edges"][:, 1])
+        assert np.all((0 <= i["gamma"]) & (i["gamma"] <= np.pi)) and i["gamma"].size == i["p"]
+
+
+def test_surrogate_features():
+    w = np.array([0.3, 0.9, 0.1])
+    gf = surrogate.graph_features(w)
+    assert gf.shape == (120,) and np.allclose(gf[:3], [0.9, 0.3, 0.1]) and not gf[3:].any()
+    X = surrogate.features(gf, np.full((2, 4), np.pi / 2), np.full((2, 4), np.pi))
+    assert X.shape == (2, 128) and np.allclose(X[:, 120:124], 0.5) and np.allclose(X[:, 124:], 1.0)
+
+
+def test_layout_roundtrip():
+    rng = np.random.default_rng(3)
+    for lt in (3, 5):
+        for kind in (MIXED_PACKED, MIXED_TILED):
+            for n in (1, 2, 4, 6):
+                rho = random_rho(n, rng)
+                assert np.allclose(from_layout(to_layout(rho, kind, lt), kind, n, lt), rho)
