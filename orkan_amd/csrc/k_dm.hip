@@ -78,6 +78,22 @@ __device__ __forceinline__ bool tile_stored(const DmPlan &p, u64 TR, u64 TC) {
     else return (TR >> (p.L - p.LB)) >= (TC >> (p.L - p.LB));
 }
 
+// LDS entry (x, y) of the updated group; permutations with phases are applied
+// here as a gather: rho'(perm a, perm b) = i^(ph[a] - ph[b]) rho(a, b)
+template <int OP>
+__device__ __forceinline__ cd lds_get(const cd *lds, const DmOp &op, int x, int y) {
+    if constexpr (OP == ORK_DM_MONO) {
+        const int k = op.k;
+        const u64 mask = spread_bits((1 << k) - 1, op.lbit, k);
+        const int ai = op.pinv[gather_bits((u64)x, op.lbit, k)], bi = op.pinv[gather_bits((u64)y, op.lbit, k)];
+        const int xs = (int)(((u64)x & ~mask) | spread_bits(ai, op.lbit, k));
+        const int ys = (int)(((u64)y & ~mask) | spread_bits(bi, op.lbit, k));
+        return cipow(lds[xs * LDS_STRIDE + ys], op.ph[ai] - op.ph[bi]);
+    } else {
+        return lds[x * LDS_STRIDE + y];
+    }
+}
+
 template <typename S, int LAYOUT, int OP>
 __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rho, DmPlan p, DmOp op) {
     __shared__ cd lds[W_MAX * LDS_STRIDE];
@@ -148,12 +164,19 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 const int y = tk & (W - 1);
                 const int x0 = (int)ins0k((u64)(tk >> LW), op.srt, k);
                 cd in[8], out[8];
-                for (int a = 0; a < K; ++a) in[a] = lds[(x0 | (int)spread_bits(a, op.lbit, k)) * LDS_STRIDE + y];
-                for (int a = 0; a < K; ++a) {
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    if (a < K) in[a] = lds[(x0 | (int)spread_bits(a, op.lbit, k)) * LDS_STRIDE + y];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
                     out[a] = cmk(0.0, 0.0);
-                    for (int b = 0; b < K; ++b) cfma(out[a], op.m[a * K + b], in[b]);
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        if (a < K && b < K) cfma(out[a], op.m[a * K + b], in[b]);
                 }
-                for (int a = 0; a < K; ++a) lds[(x0 | (int)spread_bits(a, op.lbit, k)) * LDS_STRIDE + y] = out[a];
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    if (a < K) lds[(x0 | (int)spread_bits(a, op.lbit, k)) * LDS_STRIDE + y] = out[a];
             }
             __syncthreads();
             // columns: rho <- rho U^dagger on the column index
@@ -161,12 +184,19 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 const int x = tk >> LWk;
                 const int y0 = (int)ins0k((u64)(tk & ((1 << LWk) - 1)), op.srt, k);
                 cd in[8], out[8];
-                for (int b = 0; b < K; ++b) in[b] = lds[x * LDS_STRIDE + (y0 | (int)spread_bits(b, op.lbit, k))];
-                for (int a = 0; a < K; ++a) {
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if (b < K) in[b] = lds[x * LDS_STRIDE + (y0 | (int)spread_bits(b, op.lbit, k))];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
                     out[a] = cmk(0.0, 0.0);
-                    for (int b = 0; b < K; ++b) cfma(out[a], in[b], cconj(op.m[a * K + b]));
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        if (a < K && b < K) cfma(out[a], in[b], cconj(op.m[a * K + b]));
                 }
-                for (int b = 0; b < K; ++b) lds[x * LDS_STRIDE + (y0 | (int)spread_bits(b, op.lbit, k))] = out[b];
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if (b < K) lds[x * LDS_STRIDE + (y0 | (int)spread_bits(b, op.lbit, k))] = out[b];
             }
             __syncthreads();
         }
@@ -181,26 +211,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
             const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
             const bool direct = tile_stored<LAYOUT>(p, TR, TC) && !(LAYOUT == ORK_PACKED && TR == TC && sr < sc);
             const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR);
-            if (!direct && !mirror) continue;
-            int x[2], y[2], nw = 0;
-            if (direct) { x[nw] = i * B + sr; y[nw] = j * B + sc; ++nw; }
-            if (mirror) { x[nw] = i * B + sc; y[nw] = j * B + sr; ++nw; }
-            for (int u = 0; u < nw; ++u) {
-                cd v;
-                if constexpr (OP == ORK_DM_MONO) {
-                    const int k = op.k;
-                    const u64 mask = spread_bits((1 << k) - 1, op.lbit, k);
-                    const int ao = gather_bits((u64)x[u], op.lbit, k), bo = gather_bits((u64)y[u], op.lbit, k);
-                    const int ai = op.pinv[ao], bi = op.pinv[bo];
-                    const int xs = (int)(((u64)x[u] & ~mask) | spread_bits(ai, op.lbit, k));
-                    const int ys = (int)(((u64)y[u] & ~mask) | spread_bits(bi, op.lbit, k));
-                    v = cipow(lds[xs * LDS_STRIDE + ys], op.ph[ai] - op.ph[bi]);
-                } else {
-                    v = lds[x[u] * LDS_STRIDE + y[u]];
-                }
-                if (u == 0 && direct) stc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), v);
-                else stc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(v));
-            }
+            if (direct) stc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, op, i * B + sr, j * B + sc));
+            if (mirror) stc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, op, i * B + sc, j * B + sr)));
         }
         __syncthreads();
     }

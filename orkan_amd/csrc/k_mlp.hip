@@ -52,19 +52,21 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
     const int lr = tid >> 1, lk = (tid & 1) * 16;
     const bf16 *ga = A + (m0 + lr) * (long)lda + lk;
     const bf16 *gb = B + (n0 + lr) * (long)ldb + lk;
-    uint4 ra[2], rb[2];
-    auto gload = [&](int k0) {
-        ra[0] = *(const uint4 *)(ga + k0);
-        ra[1] = *(const uint4 *)(ga + k0 + 8);
-        rb[0] = *(const uint4 *)(gb + k0);
-        rb[1] = *(const uint4 *)(gb + k0 + 8);
-    };
-    auto sstore = [&](int buf) {
-        *(uint4 *)(&As[buf][lr * KP + lk]) = ra[0];
-        *(uint4 *)(&As[buf][lr * KP + lk + 8]) = ra[1];
-        *(uint4 *)(&Bs[buf][lr * KP + lk]) = rb[0];
-        *(uint4 *)(&Bs[buf][lr * KP + lk + 8]) = rb[1];
-    };
+    uint4 ra0, ra1, rb0, rb1;
+#define GEMM_GLOAD(k0)                                  \
+    do {                                                \
+        ra0 = *(const uint4 *)(ga + (k0));              \
+        ra1 = *(const uint4 *)(ga + (k0) + 8);          \
+        rb0 = *(const uint4 *)(gb + (k0));              \
+        rb1 = *(const uint4 *)(gb + (k0) + 8);          \
+    } while (0)
+#define GEMM_SSTORE(buf)                                         \
+    do {                                                         \
+        *(uint4 *)(&As[(buf)][lr * KP + lk]) = ra0;              \
+        *(uint4 *)(&As[(buf)][lr * KP + lk + 8]) = ra1;          \
+        *(uint4 *)(&Bs[(buf)][lr * KP + lk]) = rb0;              \
+        *(uint4 *)(&Bs[(buf)][lr * KP + lk + 8]) = rb1;          \
+    } while (0)
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -74,13 +76,13 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
     const int KT = K / BK;
-    gload(0);
-    sstore(0);
+    GEMM_GLOAD(0);
+    GEMM_SSTORE(0);
     __syncthreads();
     const int l32 = lane & 31, lhi = lane >> 5;
     for (int kt = 0; kt < KT; ++kt) {
         const int buf = kt & 1;
-        if (kt + 1 < KT) gload((kt + 1) * BK);
+        if (kt + 1 < KT) GEMM_GLOAD((kt + 1) * BK);
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int kk = s * 16 + lhi * 8;
@@ -95,9 +97,11 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (kt + 1 < KT) sstore(buf ^ 1);
+        if (kt + 1 < KT) GEMM_SSTORE(buf ^ 1);
         __syncthreads();
     }
+#undef GEMM_GLOAD
+#undef GEMM_SSTORE
     // epilogue: lane holds C[row][col] with col = l32, row = 8 (r / 4) + 4 lhi + (r % 4)
 #pragma unroll
     for (int i = 0; i < 2; ++i)

@@ -133,20 +133,27 @@ struct MonoArgs {
 template <typename S>
 __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi, u64 nbase, MonoArgs m) {
     const u64 b0 = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    // fixed-bound loops with guards keep v[][] in registers (no dynamic indexing)
     cd v[SV_UNROLL][8];
     u64 base[SV_UNROLL];
 #pragma unroll
     for (int u = 0; u < SV_UNROLL; ++u) {
         const u64 b = b0 + (u64)u * SV_THREADS;
         base[u] = ins0k(b, m.pos, m.k);
-        if (b < nbase)
-            for (int j = 0; j < m.nt; ++j) v[u][j] = ldc(psi + (base[u] | m.src[j]));
+        if (b < nbase) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m.nt) v[u][j] = ldc(psi + (base[u] | m.src[j]));
+        }
     }
 #pragma unroll
     for (int u = 0; u < SV_UNROLL; ++u) {
         const u64 b = b0 + (u64)u * SV_THREADS;
-        if (b < nbase)
-            for (int j = 0; j < m.nt; ++j) stc(psi + (base[u] | m.dst[j]), cipow(v[u][j], m.ph[j]));
+        if (b < nbase) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m.nt) stc(psi + (base[u] | m.dst[j]), cipow(v[u][j], m.ph[j]));
+        }
     }
 }
 
