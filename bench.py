@@ -38,10 +38,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="sv28", choices=["sv28", "dm14"])
+    ap.add_argument("--workload", default="sv28", choices=["sv28", "dm14", "qaoa", "surrogate"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32", action="store_true", help="complex64 storage variant")
     ap.add_argument("--quick", action="store_true", help="statevector line only (no dm14 / qaoa blocks)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain copy of the state (2^n x 16 B read + written) for counter calibration")
     return ap.parse_args()
 
 
@@ -136,6 +138,12 @@ def run_sv28(args, lib):
     mval = lib.mean(ps, C.cast(obs_dev, C.POINTER(C.c_double)))
     m1 = ev.rec()
     mean_ms = ev.ms(m0, m1)
+
+    if args.copy_calibration:
+        cp = lib.orkan_state_convert(ps, 1 if args.fp32 else 0)
+        lib.orkan_sync()
+        lib.state_free(C.byref(cp))
+        log("copy calibration kernel done")
 
     rot_bytes = wl.sv_gate_bytes("rx", n, elem)
     rot_avg = float(rot_ms.mean())
@@ -247,9 +255,15 @@ def main():
             dm_args = argparse.Namespace(steps=2, warmup=1)
             result["dm14"] = bench_dm.run(dm_args, lib, Events)
             result["qaoa"] = bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)
-    else:
+    elif args.workload == "dm14":
         from orkan_amd import bench_dm
         result = {"metric": METRIC, "dm14": bench_dm.run(args, lib, Events)}
+    elif args.workload == "qaoa":
+        from orkan_amd import bench_qaoa
+        result = {"metric": METRIC, "qaoa": bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)}
+    else:
+        from orkan_amd import bench_surrogate
+        result = {"metric": METRIC, "surrogate": bench_surrogate.run(lib)}
     print(json.dumps(result))
 
 

@@ -174,13 +174,26 @@ __global__ void head_backward_kernel(const float *__restrict__ dy, const float *
     }
 }
 
-// out[k] = sum_m a[m][k] * s[m] (a bf16 [M][K]); bias grads use s = null (sum of rows)
-__global__ __launch_bounds__(256) void colsum_kernel(const bf16 *__restrict__ a, const float *__restrict__ s, int M,
-                                                    int K, float *__restrict__ out) {
+// out[k] = sum_m a[m][k] * s[m] (a bf16 [M][K]); bias grads use s = null (sum of rows).
+// Stage 1: COLSUM_CHUNKS row chunks in parallel into part[chunk][K]; stage 2 adds the
+// chunks in a fixed order (deterministic, no atomics).
+constexpr int COLSUM_CHUNKS = 64;
+__global__ __launch_bounds__(256) void colsum_part_kernel(const bf16 *__restrict__ a, const float *__restrict__ s, int M,
+                                                         int K, float *__restrict__ part) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= K) return;
+    const int rows = (M + COLSUM_CHUNKS - 1) / COLSUM_CHUNKS;
+    const int m0 = blockIdx.y * rows, m1 = min(M, m0 + rows);
+    float acc = 0.0f;
+    for (int m = m0; m < m1; ++m) acc += (float)a[(long)m * K + k] * (s ? s[m] : 1.0f);
+    part[(long)blockIdx.y * K + k] = acc;
+}
+__global__ __launch_bounds__(256) void colsum_finish_kernel(const float *__restrict__ part, int K,
+                                                           float *__restrict__ out) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= K) return;
     float acc = 0.0f;
-    for (int m = 0; m < M; ++m) acc += (float)a[(long)m * K + k] * (s ? s[m] : 1.0f);
+    for (int c = 0; c < COLSUM_CHUNKS; ++c) acc += part[(long)c * K + k];
     out[k] = acc;
 }
 
@@ -284,8 +297,12 @@ extern "C" int ork_mlp_head_backward(void *stream, const float *dy, const float 
     return 0;
 }
 
-extern "C" int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out) {
-    colsum_kernel<<<(K + 255) / 256, 256, 0, (hipStream_t)stream>>>((const bf16 *)a, s, M, K, out);
+// part: scratch of COLSUM_CHUNKS * K floats
+extern "C" int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out, float *part) {
+    dim3 g1d((K + 255) / 256, COLSUM_CHUNKS);
+    colsum_part_kernel<<<g1d, 256, 0, (hipStream_t)stream>>>((const bf16 *)a, s, M, K, part);
+    ORK_LAUNCH_CHECK();
+    colsum_finish_kernel<<<(K + 255) / 256, 256, 0, (hipStream_t)stream>>>(part, K, out);
     ORK_LAUNCH_CHECK();
     return 0;
 }

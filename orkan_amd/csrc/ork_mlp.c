@@ -21,7 +21,7 @@ int ork_mlp_to_bf16(void *stream, const float *in, void *out, long n);
 int ork_mlp_head_forward(void *stream, const void *h, const float *w, float b, int M, int K, float *y,
                          const float *target, float *dy, float *sqerr, int valid, float scale);
 int ork_mlp_head_backward(void *stream, const float *dy, const float *w, const float *pre, void *dpre, int M, int K);
-int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out);
+int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out, float *part);
 int ork_mlp_sum(void *stream, const float *v, int n, float *out);
 int ork_mlp_build_features(void *stream, float *X, const float *gf, int ng, const float *angles, int na, long r0,
                            int rows);
@@ -52,6 +52,7 @@ struct orkan_mlp {
     void *dpre, *dpre2;         /* bf16 [M][hidden] ping-pong */
     void *tA, *tB;              /* bf16 transposes [hidden][M] */
     float *y, *dy, *sq, *tgt, *scal;
+    float *colpart;             /* 64 x hidden partial column sums */
     long step;
 };
 
@@ -105,6 +106,7 @@ orkan_mlp_t *orkan_mlp_create(int in_dim, int hidden, int nhidden, int max_batch
     q->y = dmalloc(M * 4, &err); q->dy = dmalloc(M * 4, &err);
     q->sq = dmalloc(M * 4, &err); q->tgt = dmalloc(M * 4, &err);
     q->scal = dmalloc(64, &err);
+    q->colpart = dmalloc((size_t)64 * H * 4, &err);
     if (err) {
         orkan_mlp_destroy(q);
         return NULL;
@@ -134,7 +136,7 @@ void orkan_mlp_destroy(orkan_mlp_t *q) {
             if (p[i]) (void)hipFree(p[i]);
     }
     void *p[] = {q->w_out, q->g_out, q->m_out, q->v_out, q->gb_out_d, q->x32, q->x16, q->dpre, q->dpre2,
-                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal};
+                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal, q->colpart};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i)
         if (p[i]) (void)hipFree(p[i]);
     free(q);
@@ -259,7 +261,7 @@ static int backward(orkan_mlp_t *q, int rows, const float *target, float *loss) 
                                      rows, 1.0f / (float)rows);
     if (!e) e = ork_mlp_sum(st, q->sq, Mp, q->scal);          /* sum of squared errors */
     if (!e) e = ork_mlp_sum(st, q->dy, Mp, q->gb_out_d);      /* d loss / d b_out */
-    if (!e) e = ork_mlp_colsum(st, q->act[q->nl - 1], q->dy, Mp, H, q->g_out);
+    if (!e) e = ork_mlp_colsum(st, q->act[q->nl - 1], q->dy, Mp, H, q->g_out, q->colpart);
     if (!e) e = ork_mlp_head_backward(st, q->dy, q->w_out, q->pre[q->nl - 1], q->dpre, Mp, H);
     void *dcur = q->dpre, *dnext = q->dpre2;
     for (int l = q->nl - 1; l >= 0 && !e; --l) {
@@ -269,7 +271,7 @@ static int backward(orkan_mlp_t *q, int rows, const float *target, float *loss) 
         e = ork_mlp_transpose(st, xin, q->tA, Mp, L->K);
         if (!e) e = ork_mlp_transpose(st, dcur, q->tB, Mp, L->N);
         if (!e) e = ork_mlp_gemm(st, 0, q->tA, Mp, q->tB, Mp, L->K, L->N, Mp, L->g, L->N, NULL, NULL, NULL);
-        if (!e) e = ork_mlp_colsum(st, dcur, NULL, Mp, L->N, L->gb);
+        if (!e) e = ork_mlp_colsum(st, dcur, NULL, Mp, L->N, L->gb, q->colpart);
         if (!e && l > 0) {
             /* dP_{l-1} = (dP_l W_l^T) * gelu'(pre_{l-1}) */
             e = ork_mlp_gemm(st, 2, dcur, L->N, L->wb, L->N, Mp, L->K, L->N, NULL, L->K, NULL, q->pre[l - 1], dnext);

@@ -99,13 +99,13 @@ class HostEngine(Engine):
 
 
 class DeviceEngine(Engine):
-    EXTRA = {
-        "orkan_state_upload": (C.c_int, [_ffi.PS, C.c_void_p]),
-        "orkan_state_download": (C.c_int, [_ffi.PS, C.c_void_p]),
-    }
-
     def __init__(self, path, log_tile):
-        super().__init__(_ffi.bind(path, self.EXTRA), log_tile)
+        import orkan_amd
+        from orkan_amd import batch, surrogate
+        extra = dict(orkan_amd.EXTRA)
+        extra.update(batch.EXTRA)
+        extra.update(surrogate.EXTRA)
+        super().__init__(_ffi.bind(path, extra), log_tile)
 
     def write(self, st, arr):
         assert arr.size == self.length(st)
