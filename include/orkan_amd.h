@@ -55,6 +55,9 @@ void *orkan_stream(void);                 /* the library's hipStream_t on the cu
 int orkan_sync(void);                     /* wait for all enqueued work */
 int orkan_event_record(int slot);         /* record event `slot` (0..65535) on the stream */
 double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded events */
+/* tuning hook: launch shape of the density-matrix kernels (kernel 0: tile groups, 1: phases);
+ * xcd = 1 keeps consecutive groups on one XCD, lds_bytes caps workgroups per CU */
+int orkan_set_launch_shape(int kernel, int xcd, int lds_bytes);
 
 /* ------------------------------------------------------------------------------------------------
  * Batched QAOA evaluation.  One evaluation is the reference call sequence

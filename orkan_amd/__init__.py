@@ -32,6 +32,7 @@ EXTRA = {
     "orkan_sync": (C.c_int, []),
     "orkan_event_record": (C.c_int, [C.c_int]),
     "orkan_event_ms": (C.c_double, [C.c_int, C.c_int]),
+    "orkan_set_launch_shape": (C.c_int, [C.c_int, C.c_int, C.c_int]),
 }
 
 

@@ -35,6 +35,7 @@ constexpr int LDS_STRIDE = W_MAX + 1;      // padded row: column walks hit disti
 struct DmPlan {
     u64 dim;
     u64 ngroups;
+    int xcd;        // remap group ids so consecutive groups run on one XCD
     int L;          // storage tile log (tiled layout)
     int LB;         // virtual tile log
     int g;          // gate qubits at or above LB
@@ -61,6 +62,15 @@ __device__ __forceinline__ int gather_bits(u64 x, const int *pos, int k) {
     int a = 0;
     for (int j = 0; j < k; ++j) a |= (int)((x >> pos[j]) & 1) << j;
     return a;
+}
+
+// with a grid of one workgroup per group, blocks are dispatched round-robin over
+// the 8 XCDs; the remap hands consecutive groups (neighbouring tiles) to one XCD
+__device__ __forceinline__ u64 remap_group(u64 w, u64 n, int xcd) {
+    if (!xcd || gridDim.x < n) return w;
+    const u64 per = n >> 3;
+    if (w < (per << 3)) return (w & 7) * per + (w >> 3);
+    return w;
 }
 
 template <int LAYOUT>
@@ -102,7 +112,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
     const int total = W * W;
     const int tid = threadIdx.x;
 
-    for (u64 w = blockIdx.x; w < p.ngroups; w += gridDim.x) {
+    for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
+        const u64 w = remap_group(w0, p.ngroups, p.xcd);
         u64 R, C;
         tri_decode(w, R, C);
         const u64 TRb = ins0k(R, p.gpos, g), TCb = ins0k(C, p.gpos, g);
@@ -233,9 +244,9 @@ template <typename S, int LAYOUT, int MODE>
 __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rho, DmPlan p, PhaseArgs a,
                                                               const double2 *__restrict__ cs) {
     const int LB = p.LB, B = 1 << LB;
-    for (u64 w = blockIdx.x; w < p.ngroups; w += gridDim.x) {
+    for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
         u64 TR, TC;
-        tri_decode(w, TR, TC);
+        tri_decode(remap_group(w0, p.ngroups, p.xcd), TR, TC);
         for (int e = threadIdx.x; e < B * B; e += DM_THREADS) {
             const int sa = e >> LB, sb = e & (B - 1);
             const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
@@ -314,6 +325,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_finish_kernel(double *__restric
 // 32 for packed) such that the group, B * 2^g, still fits W_MAX.
 DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
     DmPlan p;
+    p.xcd = 0;
     p.dim = (u64)1 << n;
     p.L = L;
     const int cap = (layout == ORK_TILED) ? (L < n ? L : n) : (LW_MAX < n ? LW_MAX : n);
@@ -338,18 +350,27 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
     return p;
 }
 
+// Launch shapes (residency cap through unused dynamic LDS, XCD remap); set
+// through orkan_set_launch_shape, defaults from measurements on MI355X
+struct Shape { int xcd, lds; };
+Shape g_group_shape = {1, 40960};
+Shape g_phase_shape = {1, 0};
+
 unsigned grid_for(u64 ngroups) {
     const u64 cap = (u64)1 << 30;
     return (unsigned)(ngroups < cap ? ngroups : cap);
 }
 
 template <typename S, int LAYOUT>
-int launch_group(hipStream_t st, void *rho, const DmPlan &p, int opk, const DmOp &op) {
+int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     const unsigned grid = grid_for(p.ngroups);
+    p.xcd = g_group_shape.xcd;
+    const int lds = g_group_shape.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE
+                        ? g_group_shape.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
     switch (opk) {
-    case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, 0, st>>>((S *)rho, p, op); break;
-    case ORK_DM_MONO: dm_group_kernel<S, LAYOUT, ORK_DM_MONO><<<grid, DM_THREADS, 0, st>>>((S *)rho, p, op); break;
-    case ORK_DM_DENSE: dm_group_kernel<S, LAYOUT, ORK_DM_DENSE><<<grid, DM_THREADS, 0, st>>>((S *)rho, p, op); break;
+    case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
+    case ORK_DM_MONO: dm_group_kernel<S, LAYOUT, ORK_DM_MONO><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
+    case ORK_DM_DENSE: dm_group_kernel<S, LAYOUT, ORK_DM_DENSE><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
     default: return (int)hipErrorInvalidValue;
     }
     ORK_LAUNCH_CHECK();
@@ -357,15 +378,17 @@ int launch_group(hipStream_t st, void *rho, const DmPlan &p, int opk, const DmOp
 }
 
 template <int MODE>
-int launch_phase(hipStream_t st, int prec, int layout, void *rho, const DmPlan &p, const PhaseArgs &a,
+int launch_phase(hipStream_t st, int prec, int layout, void *rho, DmPlan p, const PhaseArgs &a,
                  const double2 *cs) {
     const unsigned grid = grid_for(p.ngroups);
+    p.xcd = g_phase_shape.xcd;
+    const int lds = g_phase_shape.lds;
     if (prec == ORK_F32) {
-        if (layout == ORK_PACKED) dm_phase_kernel<float2, ORK_PACKED, MODE><<<grid, DM_THREADS, 0, st>>>((float2 *)rho, p, a, cs);
-        else dm_phase_kernel<float2, ORK_TILED, MODE><<<grid, DM_THREADS, 0, st>>>((float2 *)rho, p, a, cs);
+        if (layout == ORK_PACKED) dm_phase_kernel<float2, ORK_PACKED, MODE><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
+        else dm_phase_kernel<float2, ORK_TILED, MODE><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
     } else {
-        if (layout == ORK_PACKED) dm_phase_kernel<double2, ORK_PACKED, MODE><<<grid, DM_THREADS, 0, st>>>((double2 *)rho, p, a, cs);
-        else dm_phase_kernel<double2, ORK_TILED, MODE><<<grid, DM_THREADS, 0, st>>>((double2 *)rho, p, a, cs);
+        if (layout == ORK_PACKED) dm_phase_kernel<double2, ORK_PACKED, MODE><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
+        else dm_phase_kernel<double2, ORK_TILED, MODE><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
     }
     ORK_LAUNCH_CHECK();
     return 0;
@@ -378,6 +401,14 @@ DmPlan tile_plan(int layout, int L, int n) {
 }
 
 }  // namespace
+
+extern "C" int ork_dm_set_shape(int which, int xcd, int lds_bytes) {
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024) return (int)hipErrorInvalidValue;
+    if (which == 0) g_group_shape = Shape{xcd ? 1 : 0, lds_bytes};
+    else if (which == 1) g_phase_shape = Shape{xcd ? 1 : 0, lds_bytes};
+    else return (int)hipErrorInvalidValue;
+    return 0;
+}
 
 extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, void *rho, int n, int opk,
                             const ork_mono_t *m, const double *mat) {

@@ -11,7 +11,7 @@
 namespace {
 
 constexpr int SV_THREADS = 256;
-constexpr int SV_UNROLL = 2;      // pairs per thread: two independent 32 B loads in flight
+constexpr int SV_UNROLL = 2;      // bases per thread in the permutation kernel
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 2048;  // first-stage partial sums (fixed: deterministic order)
 
@@ -76,14 +76,49 @@ __device__ __forceinline__ cd one_op(cd b, const Coef &k) {
     }
 }
 
-template <typename S, int OP>
+// Launch shape of the streaming kernels.  Measured on MI355X (28-qubit
+// states): in-place read-modify-write streams run fastest with fewer
+// workgroups resident per CU than the hardware allows (the surplus of requests
+// thrashes DRAM rows) and with consecutive blocks kept on one XCD.  Residency
+// is capped by requesting unused dynamic LDS.
+struct SvShape {
+    int xcd;   // remap block ids so consecutive chunks run on the same XCD
+    int u;     // pairs per thread (1 or 2)
+    int lds;   // dynamic LDS bytes per workgroup (0: no cap)
+};
+
+// best shape per target qubit of a 2^28-amplitude state (sweep of xcd x u x LDS cap)
+static const int kPairShapeHost[28][3] = {
+    {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248}, {1, 2, 65536}, {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248},
+    {0, 1, 81920}, {1, 1, 53248}, {1, 2, 81920}, {1, 2, 65536}, {1, 2, 81920}, {1, 2, 81920}, {1, 2, 65536},
+    {0, 1, 81920}, {0, 1, 81920}, {0, 1, 65536}, {1, 2, 81920}, {1, 2, 81920}, {1, 1, 81920}, {1, 1, 40960},
+    {1, 1, 81920}, {0, 1, 65536}, {0, 1, 65536}, {0, 1, 65536}, {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248}};
+
+SvShape pair_shape(int q) {
+    SvShape s{1, 1, 53248};
+    if (q >= 0 && q < 28) s = SvShape{kPairShapeHost[q][0], kPairShapeHost[q][1], kPairShapeHost[q][2]};
+    return s;
+}
+
+// block id -> chunk id; with xcd, consecutive chunks go to one XCD (blocks are
+// dispatched round-robin over the 8 XCDs)
+__device__ __forceinline__ u64 chunk_of(int xcd) {
+    const unsigned b = blockIdx.x;
+    if (xcd) {
+        const unsigned per = gridDim.x >> 3;
+        if (b < (per << 3)) return (u64)(b & 7) * per + (b >> 3);
+    }
+    return b;
+}
+
+template <typename S, int OP, int U, int XCD>
 __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
-    const u64 base = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
     const u64 stride = (u64)1 << q;
-    cd a[SV_UNROLL], b[SV_UNROLL];
-    u64 i0[SV_UNROLL];
+    cd a[U], b[U];
+    u64 i0[U];
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
         i0[u] = ins0(kk, q);
         if (kk < npairs) {
@@ -92,7 +127,7 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
         }
     }
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
         if (kk < npairs) {
             pair_op<OP>(a[u], b[u], k);
@@ -102,20 +137,20 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
     }
 }
 
-template <typename S, int OP>
+template <typename S, int OP, int U, int XCD>
 __global__ __launch_bounds__(SV_THREADS) void sv_one_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
-    const u64 base = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
     const u64 stride = (u64)1 << q;
-    cd b[SV_UNROLL];
-    u64 i1[SV_UNROLL];
+    cd b[U];
+    u64 i1[U];
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
         i1[u] = ins0(kk, q) + stride;
         if (kk < npairs) b[u] = ldc(psi + i1[u]);
     }
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
         if (kk < npairs) stc(psi + i1[u], one_op<OP>(b[u], k));
     }
@@ -132,7 +167,7 @@ struct MonoArgs {
 
 template <typename S>
 __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi, u64 nbase, MonoArgs m) {
-    const u64 b0 = (u64)blockIdx.x * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 b0 = chunk_of(1) * (SV_THREADS * SV_UNROLL) + threadIdx.x;
     // fixed-bound loops with guards keep v[][] in registers (no dynamic indexing)
     cd v[SV_UNROLL][8];
     u64 base[SV_UNROLL];
@@ -242,15 +277,29 @@ __global__ __launch_bounds__(RED_THREADS) void finish_kernel(double *__restrict_
         for (int i = 0; i < NV; ++i) part[NV * RED_BLOCKS + i] = acc[i];
 }
 
-template <typename S, int OP>
-void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
-    const u64 per = (u64)SV_THREADS * SV_UNROLL;
+template <typename S, int OP, int U, int XCD>
+void launch_shape(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, int lds) {
+    const u64 per = (u64)SV_THREADS * U;
     const unsigned grid = (unsigned)((npairs + per - 1) / per);
     if (OP == ORK_SV_P || OP == ORK_SV_Z || OP == ORK_SV_S || OP == ORK_SV_SDG || OP == ORK_SV_T ||
         OP == ORK_SV_TDG)
-        sv_one_kernel<S, OP><<<grid, SV_THREADS, 0, st>>>((S *)psi, npairs, q, k);
+        sv_one_kernel<S, OP, U, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
     else
-        sv_pair_kernel<S, OP><<<grid, SV_THREADS, 0, st>>>((S *)psi, npairs, q, k);
+        sv_pair_kernel<S, OP, U, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
+}
+
+template <typename S, int OP>
+void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
+    SvShape sh = pair_shape(q);
+    // the remap needs a grid of at least 8 blocks; small states use the plain order
+    if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
+    if (sh.u == 2) {
+        if (sh.xcd) launch_shape<S, OP, 2, 1>(st, psi, npairs, q, k, sh.lds);
+        else launch_shape<S, OP, 2, 0>(st, psi, npairs, q, k, sh.lds);
+    } else {
+        if (sh.xcd) launch_shape<S, OP, 1, 1>(st, psi, npairs, q, k, sh.lds);
+        else launch_shape<S, OP, 1, 0>(st, psi, npairs, q, k, sh.lds);
+    }
 }
 
 template <typename S>
@@ -316,8 +365,9 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     const u64 per = (u64)SV_THREADS * SV_UNROLL;
     const unsigned grid = (unsigned)((nbase + per - 1) / per);
     hipStream_t st = (hipStream_t)stream;
-    if (prec == ORK_F32) sv_mono_kernel<float2><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, nbase, m);
-    else sv_mono_kernel<double2><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, nbase, m);
+    const int lds = 53248;   // same residency cap as the pair kernel
+    if (prec == ORK_F32) sv_mono_kernel<float2><<<grid, SV_THREADS, lds, st>>>((float2 *)psi, nbase, m);
+    else sv_mono_kernel<double2><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
     ORK_LAUNCH_CHECK();
     return 0;
 }
