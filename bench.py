@@ -193,6 +193,49 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def sv_fp32_block(lib, layers, n, nlayers=2):
+    """FP32-storage statevector: the same brick layers on complex64 and on
+    complex128 states from |+>; rotation-kernel GB/s (8 B per amplitude) and the
+    max amplitude error against complex128."""
+    from orkan_amd import ORKAN_FP32, StateT
+    from orkan_amd import workloads as wl
+    s64, s32 = StateT(0, None, 0), StateT(0 | ORKAN_FP32, None, 0)
+    lib.state_plus(C.byref(s64), n)
+    lib.state_plus(C.byref(s32), n)
+    ev = Events(lib)
+    rot = []
+    for l in range(nlayers):
+        for op in layers[l]:
+            for st, timed in ((s64, False), (s32, True)):
+                a = ev.rec() if timed else None
+                if op[0] == "cx":
+                    lib.cx(C.byref(st), op[1], op[2])
+                else:
+                    getattr(lib, op[0])(C.byref(st), op[1], op[2])
+                if timed and op[0] != "cx":
+                    rot.append((a, ev.rec()))
+    lib.orkan_sync()
+    ms = float(np.mean([ev.ms(a, b) for a, b in rot]))
+    back = lib.orkan_state_convert(C.byref(s32), 0)
+    N = 1 << n
+    err = 0.0
+    chunk = 1 << 24
+    a64 = np.empty(N, np.complex128)
+    b64 = np.empty(N, np.complex128)
+    assert lib.orkan_state_download(C.byref(s64), a64.ctypes.data) == 0
+    assert lib.orkan_state_download(C.byref(back), b64.ctypes.data) == 0
+    for i in range(0, N, chunk):
+        err = max(err, float(np.abs(a64[i:i + chunk] - b64[i:i + chunk]).max()))
+    norm = float(np.vdot(b64, b64).real)
+    del a64, b64
+    for st in (s64, s32, back):
+        lib.state_free(C.byref(st))
+    gbs = wl.sv_gate_bytes("rx", n, elem=8) / (ms * 1e-3) / 1e9
+    return {"rotation_GB/s": round(gbs, 1), "pct_peak": round(gbs / PEAK_HBM_GBS * 100, 1),
+            "rotation_avg_ms": round(ms, 4), "layers": nlayers,
+            "max_abs_amplitude_err_vs_fp64": err, "norm_drift": abs(norm - 1.0)}
+
+
 def cpu_baseline_sv(layer, n, elem=16, budget_s=20.0):
     """The reference library (oracle/_ref, compiled from its sources) on this
     host at the same n, OpenMP over all cores: the gates of one brick layer in
@@ -251,7 +294,11 @@ def main():
         if not args.no_cpu:
             result["cpu_baseline"] = cpu_baseline_sv(sample_layer, n)
         if not args.quick:
+            from orkan_amd import workloads as wl
             from orkan_amd import bench_dm, bench_qaoa
+            log("fp32 statevector block")
+            lay, _ = wl.brick_circuit(n, 2)
+            result["fp32_storage"] = sv_fp32_block(lib, lay, n)
             dm_args = argparse.Namespace(steps=2, warmup=1)
             result["dm14"] = bench_dm.run(dm_args, lib, Events)
             result["qaoa"] = bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)

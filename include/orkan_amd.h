@@ -110,6 +110,8 @@ int orkan_mlp_train_step(orkan_mlp_t *m, int batch, const float *x, const float 
 /* gradients of the mean squared error without updating the weights */
 int orkan_mlp_grads(orkan_mlp_t *m, int batch, const float *x, const float *target, float *const *dW,
                     float *const *db, float *dw_out, float *db_out, float *loss);
+/* indices (and values) of the k largest entries of a device float array, largest first */
+int orkan_topk_f32(const float *y_dev, long n, int k, int *idx_host, float *val_host);
 /* fp32 C [M][N] = A [M][K] B [N][K]^T for bf16 device arrays (M, N multiples of 128, K of 32) */
 int orkan_mlp_gemm_test(int M, int N, int K, const void *A, const void *B, float *C);
 

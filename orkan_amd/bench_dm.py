@@ -47,7 +47,37 @@ def build_rho_layout(n, kind, g, rank=16, log_tile=5):
     return out
 
 
-def run(args, lib, Events, layouts=(2, 1), layers=None, cpu=True):
+def fp32_drift(lib, kind, st64, st32, n, L):
+    """max |rho64 - rho32|, trace drift |sum Re rho_ii - 1| and Hermiticity drift of the FP32
+    state (max |Im rho_ii|, and for tiled the diagonal-tile mirror error)"""
+    from . import ORKAN_FP32  # noqa: F401
+    a = np.empty(L, np.complex128)
+    b = np.empty(L, np.complex64)
+    assert lib.orkan_state_download(C.byref(st64), a.ctypes.data) == 0
+    assert lib.orkan_state_download(C.byref(st32), b.ctypes.data) == 0
+    D = 1 << n
+    x = np.arange(D)
+    if kind == 1:
+        diag = x * (2 * D - x + 1) // 2
+    else:
+        td = 32
+        t = x >> 5
+        diag = (t * (t + 1) // 2 + t) * td * td + (x & 31) * td + (x & 31)
+    out = {"max_abs_err": float(np.abs(a - b.astype(np.complex128)).max()),
+           "trace_drift": float(abs(np.sum(b[diag].real, dtype=np.float64) - 1.0)),
+           "trace_drift_fp64": float(abs(np.sum(a[diag].real) - 1.0)),
+           "herm_diag_imag_max": float(np.abs(b[diag].imag).max())}
+    if kind == 2:
+        worst = 0.0
+        for k in range(D >> 5):
+            off = (k * (k + 1) // 2 + k) * 1024
+            tile = b[off:off + 1024].reshape(32, 32)
+            worst = max(worst, float(np.abs(tile - tile.conj().T).max()))
+        out["herm_tile_mirror_max"] = worst
+    return out
+
+
+def run(args, lib, Events, layouts=(2, 1), layers=None, cpu=True, fp32=True):
     n = wl.DM_QUBITS
     steps = args.steps if args.steps is not None else wl.DM_LAYERS
     g = wl.rng(3)
@@ -108,5 +138,34 @@ def run(args, lib, Events, layouts=(2, 1), layers=None, cpu=True):
         res[name] = {"stored_entries": L, "GB/s": round(total_bytes / (tot * 1e-3) / 1e9, 1),
                      "ms_per_layer": round(tot / steps, 3), "kernels": kern}
         log(f"dm14 {name}: {res[name]['GB/s']} GB/s")
+        if fp32:
+            # FP32 storage: same start state, same layers; time them and compare with FP64
+            data = build_rho_layout(n, kind, wl.rng(3))
+            s64 = StateT(kind, None, 0)
+            lib.state_init(C.byref(s64), n, None)
+            assert lib.orkan_state_upload(C.byref(s64), data.ctypes.data) == 0
+            del data
+            s32 = lib.orkan_state_convert(C.byref(s64), 1)
+            p64, p32 = C.byref(s64), C.byref(s32)
+            ev = Events(lib)
+            t_ops = []
+            for l in range(args.warmup, args.warmup + steps):
+                for op in wl.dm_layer(n, l, thetas[l]):
+                    ps = p64
+                    apply(op)
+                    ps = p32
+                    a = ev.rec()
+                    apply(op)
+                    b = ev.rec()
+                    t_ops.append((op[0], a, b))
+            lib.orkan_sync()
+            tb = sum(wl.dm_gate_bytes("channel" if o in ("ampdamp", "depol") else o, L, elem=8) for o, _, _ in t_ops)
+            tt = sum(ev.ms(a, b) for _, a, b in t_ops)
+            res[name]["fp32_storage"] = {"GB/s": round(tb / (tt * 1e-3) / 1e9, 1), "layers": steps,
+                                         "drift_vs_fp64": fp32_drift(lib, kind, s64, s32, n, L)}
+            log(f"dm14 {name} fp32: {res[name]['fp32_storage']}")
+            lib.state_free(p64)
+            lib.state_free(p32)
+        ps = C.byref(st)
         lib.state_free(ps)
     return res

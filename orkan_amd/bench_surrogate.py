@@ -61,13 +61,17 @@ class Pool:
         self.dang = lib.orkan_dev_alloc((ang.nbytes + 7) // 8)
         assert lib.orkan_copy_to_device(self.dang, ang.ctypes.data, ang.nbytes) == 0
         self.dy = lib.orkan_dev_alloc((self.n * 4 + 7) // 8)
-        self.y = np.empty(self.n, np.float32)
 
     def rank(self, mlp, gf, top):
+        """indices of the `top` best predictions (largest first); self.vals holds them"""
+        import ctypes as C
         mlp.forward_candidates(gf, self.dang, self.n, self.dy)
-        assert self.lib.orkan_copy_to_host(self.y.ctypes.data, self.dy, self.y.nbytes) == 0
-        idx = np.argpartition(-self.y, top)[:top]
-        return idx[np.argsort(-self.y[idx])]
+        idx = np.empty(top, np.int32)
+        self.vals = np.empty(top, np.float32)
+        rc = self.lib.orkan_topk_f32(self.dy, self.n, top, idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                     self.vals.ctypes.data_as(C.POINTER(C.c_float)))
+        assert rc == 0
+        return idx.astype(np.int64)
 
     def close(self):
         self.lib.orkan_dev_free(self.dang)
@@ -160,7 +164,7 @@ def run(lib, train_graphs=512, per_graph=2048, test_graphs=8, ncand=1_000_000, t
         ex = simulate(sim, np.full(topk, gi), CG[top], CB[top])
         best_s = float(ex.max() / g["cmax"])
         t_s = time.perf_counter() - t0
-        top_err.append(float(np.mean(np.abs(pool.y[top] - ex / g["wsum"]))))
+        top_err.append(float(np.mean(np.abs(pool.vals - ex / g["wsum"]))))
         # plain batched search over the same pool, random order
         order = np.random.default_rng(k).permutation(ncand)[:plain_cap]
         t0 = time.perf_counter()

@@ -240,6 +240,79 @@ __global__ void build_features_kernel(float *__restrict__ X, const float *__rest
     }
 }
 
+// ---- top-k selection (radix select on order-preserving float keys) ---------------------------
+__device__ __forceinline__ unsigned fkey(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+struct TopkState {
+    unsigned prefix, mask;   // bits of the k-th largest key fixed so far
+    long need;               // elements still to take among those matching the prefix
+    unsigned hist[256];
+    unsigned n_gt, n_eq;     // compaction counters
+};
+
+__global__ __launch_bounds__(256) void topk_hist_kernel(const float *__restrict__ y, long n, TopkState *st, int shift) {
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned prefix = st->prefix, mask = st->mask;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const unsigned key = fkey(y[i]);
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// walk the bins from the largest digit: the k-th largest key has the digit where the
+// running count reaches `need`
+__global__ void topk_select_kernel(TopkState *st, int shift) {
+    if (threadIdx.x != 0) return;
+    long cum = 0;
+    int digit = 0;
+    for (int b = 255; b >= 0; --b) {
+        if (cum + (long)st->hist[b] >= st->need) {
+            digit = b;
+            break;
+        }
+        cum += st->hist[b];
+    }
+    st->need -= cum;
+    st->prefix |= (unsigned)digit << shift;
+    st->mask |= 255u << shift;
+    for (int b = 0; b < 256; ++b) st->hist[b] = 0;
+}
+
+__global__ void topk_init_kernel(TopkState *st, long k) {
+    if (threadIdx.x != 0) return;
+    st->prefix = 0;
+    st->mask = 0;
+    st->need = k;
+    st->n_gt = 0;
+    st->n_eq = 0;
+    for (int b = 0; b < 256; ++b) st->hist[b] = 0;
+}
+
+// keys above the threshold all go out; `need` of the keys equal to it (in the
+// order the atomics hand out slots -- ties are re-ordered by index on the host)
+__global__ __launch_bounds__(256) void topk_compact_kernel(const float *__restrict__ y, long n, TopkState *st,
+                                                          int k, int *__restrict__ out) {
+    const unsigned T = st->prefix;
+    const long need = st->need;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const unsigned key = fkey(y[i]);
+        if (key > T) {
+            const unsigned s = atomicAdd(&st->n_gt, 1u);
+            if (s < (unsigned)k) out[s] = (int)i;
+        } else if (key == T) {
+            const unsigned s = atomicAdd(&st->n_eq, 1u);
+            if ((long)s < need) out[k - 1 - s] = (int)i;
+        }
+    }
+}
+
 unsigned g1(long n) {
     long g = (n + 255) / 256;
     if (g > 65536) g = 65536;
@@ -317,6 +390,37 @@ extern "C" int ork_mlp_build_features(void *stream, float *X, const float *gf, i
                                       long r0, int rows) {
     build_features_kernel<<<g1((long)rows * (ng + na)), 256, 0, (hipStream_t)stream>>>(X, gf, ng, angles, na, r0,
                                                                                       rows);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+// indices of the k largest of y_dev[0..n) into idx_dev[0..k) (unordered); st_dev: scratch of
+// ork_topk_state_bytes() bytes
+extern "C" unsigned long ork_topk_state_bytes(void) { return sizeof(TopkState); }
+
+__global__ void gather_f32_kernel(const float *__restrict__ y, const int *__restrict__ idx, float *__restrict__ out,
+                                  int k) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < k) out[i] = y[idx[i]];
+}
+
+extern "C" int ork_topk(void *stream, const float *y, long n, int k, int *idx, float *vals, void *st_dev) {
+    if (k <= 0 || (long)k > n) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    TopkState *ts = (TopkState *)st_dev;
+    long g = (n + 255) / 256;
+    if (g > 2048) g = 2048;
+    topk_init_kernel<<<1, 64, 0, st>>>(ts, k);
+    ORK_LAUNCH_CHECK();
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        topk_hist_kernel<<<(unsigned)g, 256, 0, st>>>(y, n, ts, shift);
+        ORK_LAUNCH_CHECK();
+        topk_select_kernel<<<1, 64, 0, st>>>(ts, shift);
+        ORK_LAUNCH_CHECK();
+    }
+    topk_compact_kernel<<<(unsigned)g, 256, 0, st>>>(y, n, ts, k, idx);
+    ORK_LAUNCH_CHECK();
+    gather_f32_kernel<<<(k + 255) / 256, 256, 0, st>>>(y, idx, vals, k);
     ORK_LAUNCH_CHECK();
     return 0;
 }

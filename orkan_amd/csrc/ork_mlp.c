@@ -336,6 +336,51 @@ int orkan_mlp_grads(orkan_mlp_t *q, int batch, const float *x, const float *targ
     return e;
 }
 
+unsigned long ork_topk_state_bytes(void);
+int ork_topk(void *stream, const float *y, long n, int k, int *idx, float *vals, void *st_dev);
+
+/* indices of the k largest entries of the device array y_dev (n floats), largest first, ties
+ * by ascending index; values optional.  Selection on the device (radix select), the k
+ * survivors ordered on the host. */
+int orkan_topk_f32(const float *y_dev, long n, int k, int *idx_host, float *val_host) {
+    if (!y_dev || !idx_host || k <= 0 || k > n || k > 65536) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)ork_dev()->stream;
+    void *scratch = NULL;
+    int e = (int)hipMalloc(&scratch, ork_topk_state_bytes() + (size_t)k * (sizeof(int) + sizeof(float)));
+    float *vals = malloc((size_t)k * sizeof(float));
+    int *idx = malloc((size_t)k * sizeof(int));
+    int *ord = malloc((size_t)k * sizeof(int));
+    if (!vals || !idx || !ord) e = (int)hipErrorOutOfMemory;
+    int *idx_d = (int *)((char *)scratch + ork_topk_state_bytes());
+    float *val_d = (float *)(idx_d + k);
+    if (!e) e = ork_topk(st, y_dev, n, k, idx_d, val_d, scratch);
+    if (!e) e = (int)hipMemcpyAsync(idx, idx_d, (size_t)k * sizeof(int), hipMemcpyDeviceToHost, st);
+    if (!e) e = (int)hipMemcpyAsync(vals, val_d, (size_t)k * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (!e) e = (int)hipStreamSynchronize(st);
+    if (!e) {
+        /* insertion sort of k positions: value descending, then index ascending */
+        for (int i = 0; i < k; ++i) ord[i] = i;
+        for (int a = 1; a < k; ++a) {
+            const int x = ord[a];
+            int b = a - 1;
+            while (b >= 0 && (vals[ord[b]] < vals[x] || (vals[ord[b]] == vals[x] && idx[ord[b]] > idx[x]))) {
+                ord[b + 1] = ord[b];
+                --b;
+            }
+            ord[b + 1] = x;
+        }
+        for (int i = 0; i < k; ++i) {
+            idx_host[i] = idx[ord[i]];
+            if (val_host) val_host[i] = vals[ord[i]];
+        }
+    }
+    free(vals);
+    free(idx);
+    free(ord);
+    if (scratch) (void)hipFree(scratch);
+    return e;
+}
+
 /* C = A B^T for bf16 A [M][K], B [N][K] (device pointers), fp32 C [M][N]: exposed for testing the MFMA GEMM */
 int orkan_mlp_gemm_test(int M, int N, int K, const void *A, const void *B, float *C) {
     hipStream_t st = (hipStream_t)ork_dev()->stream;

@@ -23,6 +23,7 @@ EXTRA = {
     "orkan_mlp_grads": (C.c_int, [VP, C.c_int, FP, FP, C.POINTER(FP), C.POINTER(FP), FP, FP, FP]),
     "orkan_mlp_gemm_test": (C.c_int, [C.c_int, C.c_int, C.c_int, VP, VP, VP]),
     "orkan_mlp_forward_candidates": (C.c_int, [VP, FP, C.c_int, VP, C.c_long, VP]),
+    "orkan_topk_f32": (C.c_int, [VP, C.c_long, C.c_int, C.POINTER(C.c_int), FP]),
 }
 
 IN_DIM, HIDDEN, LAYERS, EDGE_SLOTS, P = 128, 512, 3, 120, 4

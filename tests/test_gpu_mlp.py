@@ -92,3 +92,24 @@ def test_training_lowers_loss():
     final = float(np.mean((m.forward(X[:2048]) - T[:2048]) ** 2))
     assert final < 0.05 * first, (first, final)
     m.close()
+
+
+@pytest.mark.parametrize("n,k", [(1000, 1), (1000, 64), (1 << 20, 64), (5000, 5000)])
+def test_topk_matches_numpy(n, k):
+    lib = orkan_amd.load(5)
+    rng = np.random.default_rng(n + k)
+    y = rng.standard_normal(n).astype(np.float32)
+    y[rng.integers(0, n, n // 10)] = 0.5          # ties
+    y[: min(n, 7)] = -0.0                          # signed zeros
+    p = upload(lib, y)
+    idx = np.empty(k, np.int32)
+    val = np.empty(k, np.float32)
+    import ctypes as C
+    assert lib.orkan_topk_f32(p, n, k, idx.ctypes.data_as(C.POINTER(C.c_int)),
+                              val.ctypes.data_as(C.POINTER(C.c_float))) == 0
+    order = np.lexsort((np.arange(n), -y))          # value descending, index ascending
+    want_vals = y[order[:k]]
+    assert np.array_equal(val, want_vals)
+    assert np.array_equal(y[idx], val)
+    assert len(set(idx.tolist())) == k
+    lib.orkan_dev_free(p)
