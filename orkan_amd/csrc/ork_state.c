@@ -93,7 +93,10 @@ void state_init(state_t *state, qubit_t qubits, cplx_t **data) {
                 *data = NULL;
                 return;
             }
-            ORK_HIP("state_init upload", hipMemcpy(state->data, *data, bytes, hipMemcpyHostToDevice));
+            /* on the library stream, after the zero fill that dev_zeros enqueued there */
+            hipStream_t st = (hipStream_t)ork_dev()->stream;
+            ORK_HIP("state_init upload", hipMemcpyAsync(state->data, *data, bytes, hipMemcpyHostToDevice, st));
+            ORK_HIP("state_init upload", hipStreamSynchronize(st));
             free(*data);
         }
         *data = NULL;
@@ -154,29 +157,32 @@ static idx_t tiled_index(idx_t r, idx_t c) {
 
 static idx_t packed_index(idx_t dim, idx_t r, idx_t c) { return c * (2 * dim - c + 1) / 2 + (r - c); }
 
+/* one element through the library stream: ordered after every enqueued kernel */
 static cplx_t elem_read(const state_t *s, idx_t at) {
-    ork_dev_t *d = ork_dev();
-    ORK_HIP("state_get", hipStreamSynchronize((hipStream_t)d->stream));
+    hipStream_t st = (hipStream_t)ork_dev()->stream;
     if (ork_prec(s) == ORK_F32) {
         float v[2];
-        ORK_HIP("state_get", hipMemcpy(v, (char *)s->data + at * 8, 8, hipMemcpyDeviceToHost));
+        ORK_HIP("state_get", hipMemcpyAsync(v, (char *)s->data + at * 8, 8, hipMemcpyDeviceToHost, st));
+        ORK_HIP("state_get", hipStreamSynchronize(st));
         return CMPLX((double)v[0], (double)v[1]);
     }
     double v[2];
-    ORK_HIP("state_get", hipMemcpy(v, (char *)s->data + at * 16, 16, hipMemcpyDeviceToHost));
+    ORK_HIP("state_get", hipMemcpyAsync(v, (char *)s->data + at * 16, 16, hipMemcpyDeviceToHost, st));
+    ORK_HIP("state_get", hipStreamSynchronize(st));
     return CMPLX(v[0], v[1]);
 }
 
 static void elem_write(state_t *s, idx_t at, cplx_t val) {
-    ork_dev_t *d = ork_dev();
-    ORK_HIP("state_set", hipStreamSynchronize((hipStream_t)d->stream));
+    hipStream_t st = (hipStream_t)ork_dev()->stream;
     if (ork_prec(s) == ORK_F32) {
         float v[2] = {(float)creal(val), (float)cimag(val)};
-        ORK_HIP("state_set", hipMemcpy((char *)s->data + at * 8, v, 8, hipMemcpyHostToDevice));
+        ORK_HIP("state_set", hipMemcpyAsync((char *)s->data + at * 8, v, 8, hipMemcpyHostToDevice, st));
+        ORK_HIP("state_set", hipStreamSynchronize(st));
         return;
     }
     double v[2] = {creal(val), cimag(val)};
-    ORK_HIP("state_set", hipMemcpy((char *)s->data + at * 16, v, 16, hipMemcpyHostToDevice));
+    ORK_HIP("state_set", hipMemcpyAsync((char *)s->data + at * 16, v, 16, hipMemcpyHostToDevice, st));
+    ORK_HIP("state_set", hipStreamSynchronize(st));
 }
 
 cplx_t state_get(const state_t *state, idx_t row, idx_t col) {

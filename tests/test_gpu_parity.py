@@ -156,3 +156,23 @@ def test_generic_matrices(pair):
                     orc.api.two_from_mat(C.byref(so), a, b, m.ctypes.data)
                     _same(dev, orc, sd, so, ("two_from_mat", a, b))
             dev.free(sd); orc.free(so)
+
+
+def test_state_init_takes_ownership_of_host_buffer(pair):
+    """state_init(s, n, &buf) with a malloc'd host buffer: the data lands in the state,
+    the library frees the buffer and nulls the caller's pointer (src/state/state.c:60-99)."""
+    import ctypes as C
+    from orkan_amd import _ffi
+    dev, orc = pair
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    for kind, n in ((PURE, 5), (MIXED_TILED, 4), (MIXED_PACKED, 3)):
+        data = _data(kind, n, 7, dev.log_tile)
+        buf = C.c_void_p(libc.malloc(data.nbytes))
+        C.memmove(buf.value, data.ctypes.data, data.nbytes)
+        st = _ffi.StateT(kind, None, 0)
+        dev.api.state_init(C.byref(st), n, C.byref(buf))
+        assert buf.value is None
+        assert np.array_equal(dev.read(st), data)
+        dev.free(st)
