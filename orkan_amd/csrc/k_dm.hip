@@ -352,8 +352,10 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 
 // Launch shapes (residency cap through unused dynamic LDS, XCD remap); set
 // through orkan_set_launch_shape, defaults from measurements on MI355X
+// (measured: unlike the statevector streams these kernels want every resident
+// workgroup they can get; the XCD remap helps the tiled layout only)
 struct Shape { int xcd, lds; };
-Shape g_group_shape = {1, 40960};
+Shape g_group_shape[2] = {{0, 0}, {1, 0}};   // [packed, tiled]
 Shape g_phase_shape = {1, 0};
 
 unsigned grid_for(u64 ngroups) {
@@ -364,9 +366,9 @@ unsigned grid_for(u64 ngroups) {
 template <typename S, int LAYOUT>
 int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     const unsigned grid = grid_for(p.ngroups);
-    p.xcd = g_group_shape.xcd;
-    const int lds = g_group_shape.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE
-                        ? g_group_shape.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
+    const Shape sh = g_group_shape[LAYOUT == ORK_TILED ? 1 : 0];
+    p.xcd = sh.xcd;
+    const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
     switch (opk) {
     case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
     case ORK_DM_MONO: dm_group_kernel<S, LAYOUT, ORK_DM_MONO><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
@@ -404,7 +406,7 @@ DmPlan tile_plan(int layout, int L, int n) {
 
 extern "C" int ork_dm_set_shape(int which, int xcd, int lds_bytes) {
     if (lds_bytes < 0 || lds_bytes > 160 * 1024) return (int)hipErrorInvalidValue;
-    if (which == 0) g_group_shape = Shape{xcd ? 1 : 0, lds_bytes};
+    if (which == 0) g_group_shape[0] = g_group_shape[1] = Shape{xcd ? 1 : 0, lds_bytes};
     else if (which == 1) g_phase_shape = Shape{xcd ? 1 : 0, lds_bytes};
     else return (int)hipErrorInvalidValue;
     return 0;

@@ -125,6 +125,95 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
             }
 }
 
+// ---- 256 x 256 tile variant: 4 waves of 128 x 128 (4 x 4 MFMA tiles each) ---------------------
+// Twice the FLOPs per LDS byte of the 128 x 128 kernel (each operand fragment read from
+// LDS feeds four MFMAs instead of two); used when M and N are multiples of 256.
+constexpr int BM2 = 256, BN2 = 256;
+
+template <int EPI>
+__global__ __launch_bounds__(GT) void gemm_nt_big_kernel(const bf16 *__restrict__ A, int lda,
+                                                         const bf16 *__restrict__ B, int ldb, int K, EpiArgs ep) {
+    __shared__ __attribute__((aligned(16))) bf16 As[2][BM2 * KP];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN2 * KP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long m0 = (long)blockIdx.y * BM2, n0 = (long)blockIdx.x * BN2;
+    // staging: each thread moves one 32-element row slice of A and of B per K step
+    const int lr = tid;                       // row 0..255
+    const bf16 *ga = A + (m0 + lr) * (long)lda;
+    const bf16 *gb = B + (n0 + lr) * (long)ldb;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define BIG_GLOAD(k0)                                                              \
+    do {                                                                           \
+        ra0 = *(const uint4 *)(ga + (k0));      ra1 = *(const uint4 *)(ga + (k0) + 8);  \
+        ra2 = *(const uint4 *)(ga + (k0) + 16); ra3 = *(const uint4 *)(ga + (k0) + 24); \
+        rb0 = *(const uint4 *)(gb + (k0));      rb1 = *(const uint4 *)(gb + (k0) + 8);  \
+        rb2 = *(const uint4 *)(gb + (k0) + 16); rb3 = *(const uint4 *)(gb + (k0) + 24); \
+    } while (0)
+#define BIG_SSTORE(buf)                                                            \
+    do {                                                                           \
+        bf16 *pa = &As[(buf)][lr * KP], *pb = &Bs[(buf)][lr * KP];                  \
+        *(uint4 *)(pa) = ra0; *(uint4 *)(pa + 8) = ra1;                            \
+        *(uint4 *)(pa + 16) = ra2; *(uint4 *)(pa + 24) = ra3;                      \
+        *(uint4 *)(pb) = rb0; *(uint4 *)(pb + 8) = rb1;                            \
+        *(uint4 *)(pb + 16) = rb2; *(uint4 *)(pb + 24) = rb3;                      \
+    } while (0)
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int KT = K / BK;
+    BIG_GLOAD(0);
+    BIG_SSTORE(0);
+    __syncthreads();
+    const int l32 = lane & 31, lhi = lane >> 5;
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) BIG_GLOAD((kt + 1) * BK);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int kk = s * 16 + lhi * 8;
+            bf16x8 av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = *(const bf16x8 *)(&As[buf][(wm * 128 + i * 32 + l32) * KP + kk]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = *(const bf16x8 *)(&Bs[buf][(wn * 128 + j * 32 + l32) * KP + kk]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < KT) BIG_SSTORE(buf ^ 1);
+        __syncthreads();
+    }
+#undef BIG_GLOAD
+#undef BIG_SSTORE
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long row = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
+                const long col = n0 + wn * 128 + j * 32 + l32;
+                const long at = row * ep.ldc + col;
+                const float v = acc[i][j][r];
+                if constexpr (EPI == EPI_F32) ep.c32[at] = v;
+                else if constexpr (EPI == EPI_F32_ACC) ep.c32[at] += v;
+                else if constexpr (EPI == EPI_BIAS_GELU) {
+                    const float z = v + ep.bias[col];
+                    if (ep.pre) ep.pre[at] = z;
+                    ep.act[at] = (bf16)gelu(z);
+                } else {
+                    ep.act[at] = (bf16)(v * gelu_grad(ep.pre[at]));
+                }
+            }
+}
+
 // ---- small kernels -----------------------------------------------------------------------------
 __global__ void transpose_bf16_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int R, int Cc) {
     __shared__ bf16 t[32][33];
@@ -323,14 +412,29 @@ unsigned g1(long n) {
 }  // namespace
 
 // ---- C ABI ------------------------------------------------------------------------------------
+static int g_gemm_big = 1;   // use the 256 x 256 tile where the shape allows
+extern "C" int ork_mlp_gemm_select(int big) { g_gemm_big = big ? 1 : 0; return 0; }
+
 // epi: 0 fp32 C, 1 bias+GELU, 2 GELU backward, 3 fp32 C +=
 extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
                             float *c32, int ldc, const float *bias, float *pre, void *act) {
     if (M % BM || N % BN || K % BK || M <= 0 || N <= 0 || K <= 0) return (int)hipErrorInvalidValue;
     EpiArgs ep{c32, ldc, bias, pre, (bf16 *)act};
-    dim3 grid(N / BN, M / BM);
     hipStream_t st = (hipStream_t)stream;
     const bf16 *a = (const bf16 *)A, *b = (const bf16 *)B;
+    if (g_gemm_big && M % BM2 == 0 && N % BN2 == 0) {
+        dim3 grid2(N / BN2, M / BM2);
+        switch (epi) {
+        case EPI_F32: gemm_nt_big_kernel<EPI_F32><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_BIAS_GELU: gemm_nt_big_kernel<EPI_BIAS_GELU><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_GELU_BWD: gemm_nt_big_kernel<EPI_GELU_BWD><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_F32_ACC: gemm_nt_big_kernel<EPI_F32_ACC><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        default: return (int)hipErrorInvalidValue;
+        }
+        ORK_LAUNCH_CHECK();
+        return 0;
+    }
+    dim3 grid(N / BN, M / BM);
     switch (epi) {
     case EPI_F32: gemm_nt_kernel<EPI_F32><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
     case EPI_BIAS_GELU: gemm_nt_kernel<EPI_BIAS_GELU><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;

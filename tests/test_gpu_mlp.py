@@ -28,7 +28,8 @@ def upload(lib, arr):
     return p
 
 
-@pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 96), (512, 128, 512)])
+@pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 96), (512, 128, 512), (256, 256, 64),
+                                   (512, 512, 512)])
 def test_gemm_exact_small_integers(M, N, K):
     lib = orkan_amd.load(5)
     rng = np.random.default_rng(M + N + K)
