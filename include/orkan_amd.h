@@ -55,9 +55,13 @@ void *orkan_stream(void);                 /* the library's hipStream_t on the cu
 int orkan_sync(void);                     /* wait for all enqueued work */
 int orkan_event_record(int slot);         /* record event `slot` (0..65535) on the stream */
 double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded events */
-/* tuning hook: launch shape of the density-matrix kernels (kernel 0: tile groups, 1: phases);
- * xcd = 1 keeps consecutive groups on one XCD, lds_bytes caps workgroups per CU */
-int orkan_set_launch_shape(int kernel, int xcd, int lds_bytes);
+/* tuning hook: launch shape of the density-matrix kernels (kernel 0: tile groups of both layouts,
+ * 1: phases of both layouts, 2 / 3: tile groups of the packed / tiled layout, 4 / 5: phases of the
+ * packed / tiled layout, 6 / 7: streamed tiled 1-qubit superoperator with the target inside / above
+ * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups);
+ * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column;
+ * lds_bytes caps workgroups per CU (0: no cap) */
+int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);
 
 /* ------------------------------------------------------------------------------------------------
  * Batched QAOA evaluation.  One evaluation is the reference call sequence

@@ -2,20 +2,23 @@
 
 1. Training data: `train_graphs` G(16, 0.5) MaxCut instances (weights U[0, 1]),
    `per_graph` random angle sets each (gamma, beta ~ U[0, pi]); energies from
-   the batched simulator, target <C> / sum(w).
-2. Training on the GPU (BF16 MFMA forward/backward, Adam), then `rounds` of
-   refinement on the training graphs only: the surrogate ranks a candidate pool,
-   the simulator labels its top candidates, and those labels join the training
-   set -- the model learns the region it will be asked to rank.
-3. Held-out graphs: MAE against the simulator on uniform angles and on the
-   surrogate's own top picks.
+   the batched simulator.  The regression target is <C> / sum(w), standardised
+   with the training set's mean and standard deviation (the raw target varies
+   by a few percent around 0.5, which an MSE fit with BF16 activations resolves
+   poorly; standardising is an affine map, so rankings are unchanged).
+2. Training on the GPU: BF16 MFMA forward/backward and Adam, `epochs` passes
+   over the data with an exponentially decaying learning rate.
+3. Held-out graphs: MAE and Spearman rank correlation against the simulator on
+   uniform angles, and MAE on the surrogate's own top picks.
 4. Search per held-out graph over one pool of `ncand` random angle sets:
      screened -- surrogate on the whole pool, exact simulation of its top 64;
      plain    -- exact simulation of the pool in a random order, in batches,
-                 until its best ratio reaches the screened one.
-   The plain time counts only the evaluations needed (index of the first
-   candidate reaching the ratio times the steady-state time per evaluation),
-   which favours the plain search.  Approximation ratio = <C> / max_x C(x).
+                 until its best ratio reaches the screened one; it always
+                 does, at the latest at the screened pick itself.
+   The whole pool is simulated once per graph, which gives the exact expected
+   number of plain evaluations to reach the screened ratio and the
+   steady-state time per evaluation; the plain time counts only the expected
+   evaluations, not whole batches.  Approximation ratio = <C> / max_x C(x).
 """
 import sys
 import time
@@ -78,9 +81,8 @@ class Pool:
         self.lib.orkan_dev_free(self.dy)
 
 
-def run(lib, train_graphs=512, per_graph=2048, test_graphs=8, ncand=1_000_000, topk=64, epochs=4,
-        batch_size=4096, plain_batch=16384, rounds=3, ref_cand=65536, ref_top=256, ref_steps=400,
-        plain_cap=1 << 20):
+def run(lib, train_graphs=512, per_graph=2048, test_graphs=8, ncand=1_000_000, topk=64, epochs=24,
+        batch_size=4096, plain_batch=65536, lr=(1e-3, 1e-4)):
     rng = wl.rng(5)
     tr = graphs(rng, train_graphs)
     te = graphs(rng, test_graphs)
@@ -89,7 +91,7 @@ def run(lib, train_graphs=512, per_graph=2048, test_graphs=8, ncand=1_000_000, t
     wsum = np.array([g["wsum"] for g in allg])
     gfs = np.stack([g["gf"] for g in allg])
 
-    # ---- 1. uniform training data from the simulator
+    # ---- 1. training data from the simulator
     t0 = time.perf_counter()
     gidx = np.repeat(np.arange(train_graphs), per_graph)
     G = rng.uniform(0, np.pi, (gidx.size, P))
@@ -98,105 +100,84 @@ def run(lib, train_graphs=512, per_graph=2048, test_graphs=8, ncand=1_000_000, t
     t_data = time.perf_counter() - t0
     log(f"surrogate: {gidx.size} training energies in {t_data:.1f} s")
 
+    # ---- 2. training
+    T = E / wsum[gidx]
+    mu, sd = float(T.mean()), float(T.std())
+    T = ((T - mu) / sd).astype(np.float32)
     mlp = surrogate.Mlp(lib, max_batch=65536)
     mlp.init(rng)
-
-    def train(gi, GG, BB, EE, steps, lr0):
-        T = (EE / wsum[gi]).astype(np.float32)
-        losses = []
-        for s in range(steps):
-            idx = rng.integers(0, gi.size, batch_size)
-            X = surrogate.features(gfs[gi[idx]], GG[idx], BB[idx])
-            losses.append(mlp.train_step(X, T[idx], lr0 * (0.1 ** (s / max(steps - 1, 1)))))
-        return losses
-
-    # ---- 2. training and refinement rounds
     t0 = time.perf_counter()
-    steps0 = epochs * (gidx.size // batch_size)
-    losses = train(gidx, G, B, E, steps0, 1e-3)
-    log(f"surrogate: uniform phase {steps0} steps, loss {np.mean(losses[-32:]):.3e}")
-    RG = rng.uniform(0, np.pi, (ref_cand, P))
-    RB = rng.uniform(0, np.pi, (ref_cand, P))
-    rpool = Pool(lib, RG, RB)
-    xg, xG, xB, xE = [gidx], [G], [B], [E]
-    for r in range(rounds):
-        picks_g, picks_i = [], []
-        for g in range(train_graphs):
-            top = rpool.rank(mlp, gfs[g], ref_top)
-            picks_g.append(np.full(top.size, g))
-            picks_i.append(top)
-        pg = np.concatenate(picks_g)
-        pi = np.concatenate(picks_i)
-        pE = simulate(sim, pg, RG[pi], RB[pi])
-        xg.append(pg); xG.append(RG[pi]); xB.append(RB[pi]); xE.append(pE)
-        allg_i, allG, allB, allE = (np.concatenate(v) for v in (xg, xG, xB, xE))
-        # half of every batch from the refined labels, half uniform
-        nref = allg_i.size - gidx.size
-        mix = np.concatenate([rng.integers(0, gidx.size, nref), np.arange(gidx.size, allg_i.size)])
-        losses = train(allg_i[mix], allG[mix], allB[mix], allE[mix], ref_steps, 3e-4)
-        log(f"surrogate: refinement round {r + 1}: {pg.size} new labels, loss {np.mean(losses[-32:]):.3e}")
-        # fresh pool for the next round so the labels keep probing new candidates
-        rpool.close()
-        RG = rng.uniform(0, np.pi, (ref_cand, P))
-        RB = rng.uniform(0, np.pi, (ref_cand, P))
-        rpool = Pool(lib, RG, RB)
-    rpool.close()
+    steps = max(1, int(epochs * gidx.size) // batch_size)
+    losses = []
+    for s in range(steps):
+        idx = rng.integers(0, gidx.size, batch_size)
+        X = surrogate.features(gfs[gidx[idx]], G[idx], B[idx])
+        losses.append(mlp.train_step(X, T[idx], lr[0] * (lr[1] / lr[0]) ** (s / max(steps - 1, 1))))
+        if (s + 1) % 1024 == 0:
+            log(f"surrogate: step {s + 1}/{steps}, loss {np.mean(losses[-128:]):.4f}")
     t_train = time.perf_counter() - t0
+
+    def predict(X):
+        return mlp.forward(X) * sd + mu
 
     # ---- 3. held-out accuracy (uniform angles)
     hg = np.repeat(np.arange(test_graphs), 4096) + train_graphs
     HG = rng.uniform(0, np.pi, (hg.size, P))
     HB = rng.uniform(0, np.pi, (hg.size, P))
     Eh = simulate(sim, hg, HG, HB)
-    pred = mlp.forward(surrogate.features(gfs[hg], HG, HB))
+    pred = predict(surrogate.features(gfs[hg], HG, HB))
     mae = float(np.mean(np.abs(pred - Eh / wsum[hg])))
     mae_energy = float(np.mean(np.abs(pred * wsum[hg] - Eh)))
+    spear = float(np.mean([spearman(pred[hg == g], Eh[hg == g]) for g in np.unique(hg)]))
 
     # ---- 4. search on the held-out graphs over one shared candidate pool
     CG = rng.uniform(0, np.pi, (ncand, P))
     CB = rng.uniform(0, np.pi, (ncand, P))
     pool = Pool(lib, CG, CB)
+    pool.rank(mlp, gfs[0], topk)                         # warm-up of the screening path
     rows, top_err = [], []
     for k, g in enumerate(te):
         gi = train_graphs + k
-        t0 = time.perf_counter()
+        ts0 = time.perf_counter()
         top = pool.rank(mlp, g["gf"], topk)
+        t_rank = time.perf_counter() - ts0
         ex = simulate(sim, np.full(topk, gi), CG[top], CB[top])
         best_s = float(ex.max() / g["cmax"])
-        t_s = time.perf_counter() - t0
-        top_err.append(float(np.mean(np.abs(pool.vals - ex / g["wsum"]))))
-        # plain batched search over the same pool, random order
-        order = np.random.default_rng(k).permutation(ncand)[:plain_cap]
+        t_s = time.perf_counter() - ts0
+        top_err.append(float(np.mean(np.abs(pool.vals * sd + mu - ex / g["wsum"]))))
+        # plain batched search: the whole pool, exactly, in batches.  Visiting the
+        # pool in a uniformly random order, the number of evaluations until the
+        # first candidate at least as good as the screened pick is expected to be
+        # (N + 1) / (K + 1), K = candidates at or above that ratio (the screened
+        # pick among them); the time is that count times the measured time per
+        # evaluation.
         t0 = time.perf_counter()
-        best_p, first, done = 0.0, None, 0
-        while done < order.size:
-            sel = order[done:done + plain_batch]
-            e = simulate(sim, np.full(sel.size, gi), CG[sel], CB[sel]) / g["cmax"]
-            if first is None and e.max() >= best_s - 1e-12:
-                first = done + int(np.argmax(e >= best_s - 1e-12))
-            best_p = max(best_p, float(e.max()))
-            done += sel.size
-            if first is not None:
-                break
-        t_p_meas = time.perf_counter() - t0
-        per_eval = t_p_meas / done
-        reached = first is not None
-        t_p = (first + 1) * per_eval if reached else t_p_meas
-        rows.append({"graph": k, "best_ratio_screened": round(best_s, 6), "best_ratio_plain": round(best_p, 6),
-                     "screened_s": round(t_s, 5), "plain_s": round(t_p, 5), "plain_s_measured": round(t_p_meas, 5),
-                     "plain_evals_to_reach": (first + 1) if reached else None, "plain_evals_run": done,
-                     "reached": reached, "speedup": round(t_p / t_s, 2) if reached else None})
-        log(f"surrogate: graph {k}: screened {best_s:.4f} in {t_s * 1e3:.1f} ms; plain reaches it after "
-            f"{(first + 1) if reached else 'no'} evals ({t_p * 1e3:.1f} ms), plain best {best_p:.4f}")
+        ratios = simulate(sim, np.full(ncand, gi), CG, CB, chunk=plain_batch) / g["cmax"]
+        per_eval = (time.perf_counter() - t0) / ncand
+        K = int(np.count_nonzero(ratios >= best_s - 1e-12))
+        evals = (ncand + 1) / (K + 1)
+        t_p = evals * per_eval
+        rows.append({"graph": k, "best_ratio_screened": round(best_s, 6), "pool_best_ratio": round(float(ratios.max()), 6),
+                     "pool_at_or_above_screened": K, "screened_s": round(t_s, 5), "screen_rank_s": round(t_rank, 5),
+                     "plain_expected_evals": round(evals, 1), "plain_us_per_eval": round(per_eval * 1e6, 3),
+                     "plain_s": round(t_p, 5), "speedup": round(t_p / t_s, 1)})
+        log(f"surrogate: graph {k}: screened {best_s:.4f} in {t_s * 1e3:.1f} ms (ranking {t_rank * 1e3:.1f} ms), "
+            f"pool best {ratios.max():.4f}, {K} pool candidates at or above it; plain: {evals:.0f} evals expected "
+            f"({t_p * 1e3:.1f} ms)")
     pool.close()
     mlp.close()
     sim.close()
-    sp = np.array([r["speedup"] for r in rows if r["speedup"] is not None])
-    return {"train_samples_uniform": int(gidx.size), "refined_labels": int(sum(v.size for v in xg[1:])),
-            "train_data_s": round(t_data, 2), "train_s": round(t_train, 2),
+    sp = np.array([r["speedup"] for r in rows])
+    return {"train_samples": int(gidx.size), "train_steps": steps, "train_data_s": round(t_data, 2),
+            "train_s": round(t_train, 2), "target_mean": round(mu, 6), "target_std": round(sd, 6),
             "heldout_mae_normalised": round(mae, 5), "heldout_mae_energy": round(mae_energy, 5),
+            "heldout_mae_over_target_std": round(mae / sd, 3), "heldout_spearman": round(spear, 4),
             "heldout_mae_top_picks_normalised": round(float(np.mean(top_err)), 5),
             "candidates_per_graph": ncand, "topk": topk, "search": rows,
-            "speedup_median": float(np.median(sp)) if sp.size else None,
-            "speedup_min": float(sp.min()) if sp.size else None,
-            "all_reached_equal_ratio": all(r["reached"] for r in rows)}
+            "speedup_median": float(np.median(sp)), "speedup_min": float(sp.min())}
+
+
+def spearman(a, b):
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    return float(np.corrcoef(ra, rb)[0, 1])

@@ -35,7 +35,9 @@ constexpr int LDS_STRIDE = W_MAX + 1;      // padded row: column walks hit disti
 struct DmPlan {
     u64 dim;
     u64 ngroups;
+    u64 nt;         // group rows (dim >> LW)
     int xcd;        // remap group ids so consecutive groups run on one XCD
+    int col;        // group order: 0 row by row (R outer), 1 column by column (C outer)
     int L;          // storage tile log (tiled layout)
     int LB;         // virtual tile log
     int g;          // gate qubits at or above LB
@@ -71,6 +73,21 @@ __device__ __forceinline__ u64 remap_group(u64 w, u64 n, int xcd) {
     const u64 per = n >> 3;
     if (w < (per << 3)) return (w & 7) * per + (w >> 3);
     return w;
+}
+
+// group id -> (R, C), R >= C.  Column order walks a column of groups (same C,
+// R ascending), which for the column-major packed layout makes neighbouring
+// groups share the cache lines at the ends of their column runs.
+__device__ __forceinline__ void group_rc(const DmPlan &p, u64 w0, u64 &R, u64 &C) {
+    const u64 w = remap_group(w0, p.ngroups, p.xcd);
+    if (!p.col) {
+        tri_decode(w, R, C);
+    } else {
+        u64 rr, cc;
+        tri_decode(p.ngroups - 1 - w, rr, cc);
+        R = p.nt - 1 - cc;
+        C = p.nt - 1 - rr;
+    }
 }
 
 template <int LAYOUT>
@@ -113,9 +130,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
     const int tid = threadIdx.x;
 
     for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
-        const u64 w = remap_group(w0, p.ngroups, p.xcd);
         u64 R, C;
-        tri_decode(w, R, C);
+        group_rc(p, w0, R, C);
         const u64 TRb = ins0k(R, p.gpos, g), TCb = ins0k(C, p.gpos, g);
         const bool diag_group = (R == C);
 
@@ -229,6 +245,112 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
     }
 }
 
+// ---- tiled layout, 1-qubit superoperator, streamed ---------------------------------------
+// The quad (r0, r1) x (c0, c1), r1 = r0 | 2^q, c1 = c0 | 2^q, goes from HBM to
+// registers and back with no LDS round trip wherever its four entries are
+// stored as themselves:
+//   q <  L: the quad lies inside one stored tile (diagonal tiles hold both
+//           triangles), so the tile array is one flat stream of quads;
+//   q >= L: the quad takes the same in-tile entry of the four tiles
+//           (TR|a, TC|c), a, c in {0, b}, b = 2^(q-L); one workgroup per
+//           canonical tile quad (TR, TC with bit q-L clear, TR >= TC).  When TR
+//           and TC agree above that bit, (TR, TC|b) lies above the diagonal: it
+//           is the conjugate transpose of the stored tile X = (TC|b, TR), staged
+//           through LDS so that its reads and writes stay coalesced.  For
+//           TR == TC, X is the quad's own (TR|b, TC) tile: every quad reads X from
+//           the LDS copy and writes only its direct (r1, c0) entry, the mirror of
+//           the (r0, c1) entry another quad of the workgroup computes.
+struct StreamPlan {
+    u64 nitems;   // quads (q < L) or canonical tile quads (q >= L)
+    u64 nblocks;  // workgroups of the launch
+    int L, m;     // tile log; in-tile bit q (q < L) or tile-index bit q - L
+    int xcd;
+};
+
+__device__ __forceinline__ void sop4(const DmOp &op, cd v0, cd v1, cd v2, cd v3, cd &o0, cd &o1, cd &o2, cd &o3) {
+    o0 = cmk(0.0, 0.0); o1 = o0; o2 = o0; o3 = o0;
+    cfma(o0, op.m[0], v0); cfma(o0, op.m[1], v1); cfma(o0, op.m[2], v2); cfma(o0, op.m[3], v3);
+    cfma(o1, op.m[4], v0); cfma(o1, op.m[5], v1); cfma(o1, op.m[6], v2); cfma(o1, op.m[7], v3);
+    cfma(o2, op.m[8], v0); cfma(o2, op.m[9], v1); cfma(o2, op.m[10], v2); cfma(o2, op.m[11], v3);
+    cfma(o3, op.m[12], v0); cfma(o3, op.m[13], v1); cfma(o3, op.m[14], v2); cfma(o3, op.m[15], v3);
+}
+
+__device__ __forceinline__ u64 tile_base(u64 TR, u64 TC, int L) { return (TR * (TR + 1) / 2 + TC) << (2 * L); }
+
+template <typename S>
+__global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
+    const int L = sp.L, q = sp.m, hl = L - 1;
+    const int mb = 1 << q;
+    const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
+    const u64 Q = w * DM_THREADS + threadIdx.x;
+    if (Q >= sp.nitems) return;
+    const int i = (int)(Q & (((u64)1 << (2 * hl)) - 1));
+    const int r0 = (int)ins0((u64)(i >> hl), q), c0 = (int)ins0((u64)(i & ((1 << hl) - 1)), q);
+    S *t = rho + ((Q >> (2 * hl)) << (2 * L));
+    const int a00 = (r0 << L) | c0, a01 = a00 | mb, a10 = a00 | (mb << L), a11 = a10 | mb;
+    const cd v0 = ldc(t + a00), v1 = ldc(t + a01), v2 = ldc(t + a10), v3 = ldc(t + a11);
+    cd o0, o1, o2, o3;
+    sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+    stc(t + a00, o0);
+    stc(t + a01, o1);
+    stc(t + a10, o2);
+    stc(t + a11, o3);
+}
+
+template <typename S>
+__global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
+    __shared__ cd xs[W_MAX * LDS_STRIDE];
+    const int L = sp.L, T = 1 << L, TT = T * T, tid = threadIdx.x;
+    const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
+    if (w >= sp.nitems) return;
+    u64 Rr, Cr;
+    tri_decode(w, Rr, Cr);
+    const u64 bb = (u64)1 << sp.m;
+    const u64 TR = ins0(Rr, sp.m), TC = ins0(Cr, sp.m);
+    S *A = rho + tile_base(TR, TC, L);
+    S *C = rho + tile_base(TR | bb, TC, L);
+    S *D = rho + tile_base(TR | bb, TC | bb, L);
+    if ((TR >> (sp.m + 1)) != (TC >> (sp.m + 1))) {
+        // all four tiles stored as themselves: two in-tile entries per thread in flight
+        S *B = rho + tile_base(TR, TC | bb, L);
+        for (int o = tid; o < TT; o += 2 * DM_THREADS) {
+            const int o2 = o + DM_THREADS;
+            const bool two = o2 < TT;
+            const cd a0 = ldc(A + o), b0 = ldc(B + o), c0 = ldc(C + o), d0 = ldc(D + o);
+            cd a1 = a0, b1 = b0, c1 = c0, d1 = d0;
+            if (two) { a1 = ldc(A + o2); b1 = ldc(B + o2); c1 = ldc(C + o2); d1 = ldc(D + o2); }
+            cd p0, p1, p2, p3;
+            sop4(op, a0, b0, c0, d0, p0, p1, p2, p3);
+            stc(A + o, p0); stc(B + o, p1); stc(C + o, p2); stc(D + o, p3);
+            if (two) {
+                sop4(op, a1, b1, c1, d1, p0, p1, p2, p3);
+                stc(A + o2, p0); stc(B + o2, p1); stc(C + o2, p2); stc(D + o2, p3);
+            }
+        }
+        return;
+    }
+    const bool diag = (TR == TC);
+    S *X = diag ? C : rho + tile_base(TC | bb, TR, L);
+    for (int o = tid; o < TT; o += DM_THREADS) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldc(X + o);
+    __syncthreads();
+    for (int o = tid; o < TT; o += DM_THREADS) {
+        const int i = o >> L, j = o & (T - 1);
+        const cd v0 = ldc(A + o), v3 = ldc(D + o);
+        const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
+        const cd v2 = diag ? xs[i * LDS_STRIDE + j] : ldc(C + o);
+        cd o0, o1, o2, o3;
+        sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+        stc(A + o, o0);
+        stc(C + o, o2);
+        stc(D + o, o3);
+        if (!diag) xs[j * LDS_STRIDE + i] = cconj(o1);
+    }
+    if (!diag) {
+        __syncthreads();
+        for (int o = tid; o < TT; o += DM_THREADS) stc(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+    }
+}
+
 // ---- elementwise phase kernels: rho_rc *= F(r) conj(F(c)) --------------------------------
 // Each workgroup takes one stored B x B tile (TR >= TC).  `mode` 0: gate phases
 // from a 2^k table with a per-pair skip mask; mode 1: per-index phases (exp_diag),
@@ -246,7 +368,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
     const int LB = p.LB, B = 1 << LB;
     for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
         u64 TR, TC;
-        tri_decode(remap_group(w0, p.ngroups, p.xcd), TR, TC);
+        group_rc(p, w0, TR, TC);
         for (int e = threadIdx.x; e < B * B; e += DM_THREADS) {
             const int sa = e >> LB, sb = e & (B - 1);
             const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
@@ -326,6 +448,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_finish_kernel(double *__restric
 DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
     DmPlan p;
     p.xcd = 0;
+    p.col = 0;
     p.dim = (u64)1 << n;
     p.L = L;
     const int cap = (layout == ORK_TILED) ? (L < n ? L : n) : (LW_MAX < n ? LW_MAX : n);
@@ -346,17 +469,21 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
             if (gq[b] < gq[a]) { int t = gq[a]; gq[a] = gq[b]; gq[b] = t; }
     for (int j = 0; j < 3; ++j) p.gpos[j] = j < ng ? gq[j] : 0;
     const u64 T = p.dim >> p.LW;
+    p.nt = T;
     p.ngroups = T * (T + 1) / 2;
     return p;
 }
 
-// Launch shapes (residency cap through unused dynamic LDS, XCD remap); set
-// through orkan_set_launch_shape, defaults from measurements on MI355X
-// (measured: unlike the statevector streams these kernels want every resident
-// workgroup they can get; the XCD remap helps the tiled layout only)
-struct Shape { int xcd, lds; };
-Shape g_group_shape[2] = {{0, 0}, {1, 0}};   // [packed, tiled]
-Shape g_phase_shape = {1, 0};
+// Launch shapes (residency cap through unused dynamic LDS, XCD remap, group
+// order); set through orkan_set_launch_shape, defaults from a sweep over every
+// target qubit of a 14-qubit density matrix on MI355X.  Unlike the statevector
+// streams these kernels want every resident workgroup they can get.  Packed:
+// column order plus the XCD remap lets the groups that share the cache lines at
+// the ends of a column run meet in one L2 (without it 1.33x the algorithmic
+// bytes are fetched); tiled: row order with the remap.
+struct Shape { int xcd, col, lds; };
+Shape g_group_shape[2] = {{1, 1, 0}, {1, 0, 0}};   // [packed, tiled]
+Shape g_phase_shape[2] = {{0, 1, 0}, {0, 0, 0}};
 
 unsigned grid_for(u64 ngroups) {
     const u64 cap = (u64)1 << 30;
@@ -368,6 +495,7 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     const unsigned grid = grid_for(p.ngroups);
     const Shape sh = g_group_shape[LAYOUT == ORK_TILED ? 1 : 0];
     p.xcd = sh.xcd;
+    p.col = sh.col;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
     switch (opk) {
     case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
@@ -379,12 +507,47 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     return 0;
 }
 
+// streamed tiled superoperator; shape [q < L, q >= L].  Measured on a 14-qubit density matrix
+// (32 x 32 tiles): in-tile targets stream best like the statevector kernels, with consecutive
+// chunks on one XCD and residency capped by 64 KB of LDS per workgroup (6.1 TB/s against 5.6 TB/s
+// uncapped); tile-quad workgroups best without the remap and with an 80 KB cap (5.4 TB/s).
+Shape g_stream_shape[2] = {{1, 0, 65536}, {0, 0, 81920}};
+int g_stream_on = 1;
+
+template <typename S>
+int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op) {
+    const u64 nt = (u64)1 << (n - L);
+    StreamPlan sp;
+    sp.L = L;
+    const Shape sh = g_stream_shape[q < L ? 0 : 1];
+    sp.xcd = sh.xcd;
+    const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
+    if (q < L) {
+        sp.m = q;
+        sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 2);
+        sp.nblocks = (sp.nitems + DM_THREADS - 1) / DM_THREADS;
+        if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
+        dm_sop_intile_kernel<S><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds, st>>>((S *)rho, sp, op);
+    } else {
+        sp.m = q - L;
+        const u64 h = nt >> 1;
+        sp.nitems = h * (h + 1) / 2;
+        sp.nblocks = sp.nitems;
+        if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
+        dm_sop_tiles_kernel<S><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
+    }
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
 template <int MODE>
 int launch_phase(hipStream_t st, int prec, int layout, void *rho, DmPlan p, const PhaseArgs &a,
                  const double2 *cs) {
     const unsigned grid = grid_for(p.ngroups);
-    p.xcd = g_phase_shape.xcd;
-    const int lds = g_phase_shape.lds;
+    const Shape sh = g_phase_shape[layout == ORK_TILED ? 1 : 0];
+    p.xcd = sh.xcd;
+    p.col = sh.col;
+    const int lds = sh.lds;
     if (prec == ORK_F32) {
         if (layout == ORK_PACKED) dm_phase_kernel<float2, ORK_PACKED, MODE><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
         else dm_phase_kernel<float2, ORK_TILED, MODE><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
@@ -404,11 +567,25 @@ DmPlan tile_plan(int layout, int L, int n) {
 
 }  // namespace
 
-extern "C" int ork_dm_set_shape(int which, int xcd, int lds_bytes) {
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024) return (int)hipErrorInvalidValue;
-    if (which == 0) g_group_shape[0] = g_group_shape[1] = Shape{xcd ? 1 : 0, lds_bytes};
-    else if (which == 1) g_phase_shape = Shape{xcd ? 1 : 0, lds_bytes};
-    else return (int)hipErrorInvalidValue;
+// which: 0 group kernel (both layouts), 1 phase kernel (both layouts), 2 group kernel packed,
+// 3 group kernel tiled, 4 phase kernel packed, 5 phase kernel tiled, 6 / 7 streamed tiled
+// superoperator with the target qubit inside / above the tile, 8 streamed superoperator on (lds_bytes
+// != 0) or off; flags bit 0: XCD remap, bit 1: column order of groups
+extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 3) return (int)hipErrorInvalidValue;
+    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes};
+    switch (which) {
+    case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
+    case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;
+    case 2: g_group_shape[0] = sh; break;
+    case 3: g_group_shape[1] = sh; break;
+    case 4: g_phase_shape[0] = sh; break;
+    case 5: g_phase_shape[1] = sh; break;
+    case 6: g_stream_shape[0] = sh; break;
+    case 7: g_stream_shape[1] = sh; break;
+    case 8: g_stream_on = lds_bytes != 0; break;
+    default: return (int)hipErrorInvalidValue;
+    }
     return 0;
 }
 
@@ -444,6 +621,10 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     for (int i = 0; i < 64; ++i) op.m[i] = make_double2(0.0, 0.0);
     for (int i = 0; i < nm; ++i) op.m[i] = make_double2(mat[2 * i], mat[2 * i + 1]);
     hipStream_t st = (hipStream_t)stream;
+    if (layout == ORK_TILED && opk == ORK_DM_SOP && g_stream_on && n >= log_tile && log_tile <= LW_MAX &&
+        log_tile >= 1)
+        return prec == ORK_F32 ? launch_stream<float2>(st, rho, log_tile, n, m->qs[0], op)
+                               : launch_stream<double2>(st, rho, log_tile, n, m->qs[0], op);
     if (prec == ORK_F32)
         return layout == ORK_PACKED ? launch_group<float2, ORK_PACKED>(st, rho, p, opk, op)
                                     : launch_group<float2, ORK_TILED>(st, rho, p, opk, op);

@@ -125,66 +125,72 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
             }
 }
 
-// ---- 256 x 256 tile variant: 4 waves of 128 x 128 (4 x 4 MFMA tiles each) ---------------------
-// Twice the FLOPs per LDS byte of the 128 x 128 kernel (each operand fragment read from
-// LDS feeds four MFMAs instead of two); used when M and N are multiples of 256.
-constexpr int BM2 = 256, BN2 = 256;
+// ---- 256 x 256 tile variant: 8 waves of 128 x 64 (4 x 2 MFMA tiles each), K steps of 64 ------
+// Each operand fragment read from LDS feeds two or four MFMAs; 128 accumulator registers per
+// lane leave room for the staging registers without spills, and the 2 waves per SIMD plus the
+// 64-deep K step (2048 MFMA cycles per SIMD between barriers) cover the global-load latency of
+// the next step.  Used when M, N are multiples of 256 and K of 64.
+constexpr int BM2 = 256, BN2 = 256, BK2 = 64, KP2 = BK2 + 8, GT2 = 512;
 
 template <int EPI>
-__global__ __launch_bounds__(GT) void gemm_nt_big_kernel(const bf16 *__restrict__ A, int lda,
-                                                         const bf16 *__restrict__ B, int ldb, int K, EpiArgs ep) {
-    __shared__ __attribute__((aligned(16))) bf16 As[2][BM2 * KP];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN2 * KP];
+__global__ __launch_bounds__(GT2) void gemm_nt_big_kernel(const bf16 *__restrict__ A, int lda,
+                                                          const bf16 *__restrict__ B, int ldb, int K, EpiArgs ep) {
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];
+    bf16 *As = smem;                          // [2][BM2 * KP2]
+    bf16 *Bs = smem + 2 * BM2 * KP2;          // [2][BN2 * KP2]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves: 128 rows x 64 columns each
     const long m0 = (long)blockIdx.y * BM2, n0 = (long)blockIdx.x * BN2;
-    // staging: each thread moves one 32-element row slice of A and of B per K step
-    const int lr = tid;                       // row 0..255
-    const bf16 *ga = A + (m0 + lr) * (long)lda;
-    const bf16 *gb = B + (n0 + lr) * (long)ldb;
+    // staging: each thread moves 32 consecutive K elements of one row of A and of B per K step
+    const int lr = tid >> 1, lk = (tid & 1) * 32;
+    const bf16 *ga = A + (m0 + lr) * (long)lda + lk;
+    const bf16 *gb = B + (n0 + lr) * (long)ldb + lk;
     uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define BIG_GLOAD(k0)                                                              \
-    do {                                                                           \
+#define BIG_GLOAD(k0)                                                                   \
+    do {                                                                                \
         ra0 = *(const uint4 *)(ga + (k0));      ra1 = *(const uint4 *)(ga + (k0) + 8);  \
         ra2 = *(const uint4 *)(ga + (k0) + 16); ra3 = *(const uint4 *)(ga + (k0) + 24); \
         rb0 = *(const uint4 *)(gb + (k0));      rb1 = *(const uint4 *)(gb + (k0) + 8);  \
         rb2 = *(const uint4 *)(gb + (k0) + 16); rb3 = *(const uint4 *)(gb + (k0) + 24); \
     } while (0)
-#define BIG_SSTORE(buf)                                                            \
-    do {                                                                           \
-        bf16 *pa = &As[(buf)][lr * KP], *pb = &Bs[(buf)][lr * KP];                  \
-        *(uint4 *)(pa) = ra0; *(uint4 *)(pa + 8) = ra1;                            \
-        *(uint4 *)(pa + 16) = ra2; *(uint4 *)(pa + 24) = ra3;                      \
-        *(uint4 *)(pb) = rb0; *(uint4 *)(pb + 8) = rb1;                            \
-        *(uint4 *)(pb + 16) = rb2; *(uint4 *)(pb + 24) = rb3;                      \
+#define BIG_SSTORE(buf)                                                                  \
+    do {                                                                                 \
+        bf16 *pa = As + (buf) * BM2 * KP2 + lr * KP2 + lk;                               \
+        bf16 *pb = Bs + (buf) * BN2 * KP2 + lr * KP2 + lk;                               \
+        *(uint4 *)(pa) = ra0; *(uint4 *)(pa + 8) = ra1;                                  \
+        *(uint4 *)(pa + 16) = ra2; *(uint4 *)(pa + 24) = ra3;                            \
+        *(uint4 *)(pb) = rb0; *(uint4 *)(pb + 8) = rb1;                                  \
+        *(uint4 *)(pb + 16) = rb2; *(uint4 *)(pb + 24) = rb3;                            \
     } while (0)
-    f32x16 acc[4][4];
+    f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    const int KT = K / BK;
+    const int KT = K / BK2;
     BIG_GLOAD(0);
     BIG_SSTORE(0);
     __syncthreads();
     const int l32 = lane & 31, lhi = lane >> 5;
     for (int kt = 0; kt < KT; ++kt) {
         const int buf = kt & 1;
-        if (kt + 1 < KT) BIG_GLOAD((kt + 1) * BK);
+        if (kt + 1 < KT) BIG_GLOAD((kt + 1) * BK2);
+        const bf16 *as = As + buf * BM2 * KP2 + (wm * 128 + l32) * KP2;
+        const bf16 *bs = Bs + buf * BN2 * KP2 + (wn * 64 + l32) * KP2;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < BK2 / 16; ++s) {
             const int kk = s * 16 + lhi * 8;
-            bf16x8 av[4], bv[4];
+            bf16x8 av[4], bv[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = *(const bf16x8 *)(&As[buf][(wm * 128 + i * 32 + l32) * KP + kk]);
+            for (int i = 0; i < 4; ++i) av[i] = *(const bf16x8 *)(as + i * 32 * KP2 + kk);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = *(const bf16x8 *)(&Bs[buf][(wn * 128 + j * 32 + l32) * KP + kk]);
+            for (int j = 0; j < 2; ++j) bv[j] = *(const bf16x8 *)(bs + j * 32 * KP2 + kk);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
         if (kt + 1 < KT) BIG_SSTORE(buf ^ 1);
@@ -195,11 +201,11 @@ __global__ __launch_bounds__(GT) void gemm_nt_big_kernel(const bf16 *__restrict_
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const long row = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
-                const long col = n0 + wn * 128 + j * 32 + l32;
+                const long col = n0 + wn * 64 + j * 32 + l32;
                 const long at = row * ep.ldc + col;
                 const float v = acc[i][j][r];
                 if constexpr (EPI == EPI_F32) ep.c32[at] = v;
@@ -213,6 +219,7 @@ __global__ __launch_bounds__(GT) void gemm_nt_big_kernel(const bf16 *__restrict_
                 }
             }
 }
+constexpr int BIG_LDS = 2 * (BM2 + BN2) * KP2 * (int)sizeof(bf16);
 
 // ---- small kernels -----------------------------------------------------------------------------
 __global__ void transpose_bf16_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int R, int Cc) {
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(256) void head_forward_kernel(const bf16 *__restric
     for (int k = lane; k < K; k += 64) s += (float)h[(long)row * K + k] * w[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) {
+    if (lane == 0 && (target || row < valid)) {   // forward only: no writes past the valid rows
         const float v = s + b;
         y[row] = v;
         if (target) {
@@ -318,14 +325,59 @@ __global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, 
 }
 
 // X[r] = [gf (ng floats), angles[r0 + r] (na floats)] as fp32 rows of width ng + na
-__global__ void build_features_kernel(float *__restrict__ X, const float *__restrict__ gf, int ng,
-                                      const float *__restrict__ angles, int na, long r0, int rows) {
-    const int width = ng + na;
-    const long n = (long)rows * width;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        const long r = i / width;
-        const int c = (int)(i % width);
-        X[i] = c < ng ? gf[c] : angles[(r0 + r) * na + (c - ng)];
+// ---- first layer for screening ------------------------------------------------------------
+// Every candidate row of one graph shares the ng graph features, so their share
+// of layer 1 is computed once: u[j] = sum_{k<ng} bf16(gf[k]) W[k][j].  A row's
+// pre-activation is then (u[j] + sum_k bf16(a[r][k]) W[ng+k][j]) + b[j]: the same
+// bf16 products as the GEMM over the assembled row, summed in fp32 in another
+// order.  W is the bf16 [K][N] copy of the layer.
+__global__ void first_layer_graph_kernel(const float *__restrict__ gf, int ng, const bf16 *__restrict__ w, int N,
+                                         float *__restrict__ u) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.0f;
+    for (int k = 0; k < ng; ++k) s += (float)(bf16)gf[k] * (float)w[(long)k * N + j];
+    u[j] = s;
+}
+
+// act[r][j] = gelu(pre) for rows [0, rows) of the chunk starting at candidate r0,
+// zero for the padding rows [rows, Mp); four consecutive columns per thread
+// (one 8-byte weight load per angle, one 8-byte store).  NA > 0 fixes the angle
+// count at compile time so that all loads of a row issue together.
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+template <int NA>
+__global__ void first_layer_rows_kernel(const float *__restrict__ angles, int na_rt, long r0, int rows, int Mp,
+                                        const float *__restrict__ u, const bf16 *__restrict__ wa,
+                                        const float *__restrict__ bias, int N, bf16 *__restrict__ act) {
+    const int na = NA > 0 ? NA : na_rt;
+    const unsigned nq = (unsigned)(N >> 2);
+    const unsigned total = (unsigned)Mp * nq;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
+        const unsigned r = i / nq;
+        const int j = (int)(i - r * nq) * 4;
+        bf16x4 out;
+        if ((int)r < rows) {
+            const float4 u4 = *(const float4 *)(u + j);
+            float z0 = u4.x, z1 = u4.y, z2 = u4.z, z3 = u4.w;
+            const float *a = angles + (r0 + (long)r) * na;
+#pragma unroll 8
+            for (int k = 0; k < na; ++k) {
+                const float ak = (float)(bf16)a[k];
+                const bf16x4 w = *(const bf16x4 *)(wa + (long)k * N + j);
+                z0 += ak * (float)w[0];
+                z1 += ak * (float)w[1];
+                z2 += ak * (float)w[2];
+                z3 += ak * (float)w[3];
+            }
+            const float4 b4 = *(const float4 *)(bias + j);
+            out[0] = (bf16)gelu(z0 + b4.x);
+            out[1] = (bf16)gelu(z1 + b4.y);
+            out[2] = (bf16)gelu(z2 + b4.z);
+            out[3] = (bf16)gelu(z3 + b4.w);
+        } else {
+            out[0] = out[1] = out[2] = out[3] = (bf16)0.0f;
+        }
+        *(bf16x4 *)(act + (long)r * N + j) = out;
     }
 }
 
@@ -422,13 +474,13 @@ extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const
     EpiArgs ep{c32, ldc, bias, pre, (bf16 *)act};
     hipStream_t st = (hipStream_t)stream;
     const bf16 *a = (const bf16 *)A, *b = (const bf16 *)B;
-    if (g_gemm_big && M % BM2 == 0 && N % BN2 == 0) {
+    if (g_gemm_big && M % BM2 == 0 && N % BN2 == 0 && K % BK2 == 0) {
         dim3 grid2(N / BN2, M / BM2);
         switch (epi) {
-        case EPI_F32: gemm_nt_big_kernel<EPI_F32><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
-        case EPI_BIAS_GELU: gemm_nt_big_kernel<EPI_BIAS_GELU><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
-        case EPI_GELU_BWD: gemm_nt_big_kernel<EPI_GELU_BWD><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
-        case EPI_F32_ACC: gemm_nt_big_kernel<EPI_F32_ACC><<<grid2, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_F32: gemm_nt_big_kernel<EPI_F32><<<grid2, GT2, BIG_LDS, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_BIAS_GELU: gemm_nt_big_kernel<EPI_BIAS_GELU><<<grid2, GT2, BIG_LDS, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_GELU_BWD: gemm_nt_big_kernel<EPI_GELU_BWD><<<grid2, GT2, BIG_LDS, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_F32_ACC: gemm_nt_big_kernel<EPI_F32_ACC><<<grid2, GT2, BIG_LDS, st>>>(a, lda, b, ldb, K, ep); break;
         default: return (int)hipErrorInvalidValue;
         }
         ORK_LAUNCH_CHECK();
@@ -490,10 +542,22 @@ extern "C" int ork_mlp_sum(void *stream, const float *v, int n, float *out) {
     return 0;
 }
 
-extern "C" int ork_mlp_build_features(void *stream, float *X, const float *gf, int ng, const float *angles, int na,
-                                      long r0, int rows) {
-    build_features_kernel<<<g1((long)rows * (ng + na)), 256, 0, (hipStream_t)stream>>>(X, gf, ng, angles, na, r0,
-                                                                                      rows);
+extern "C" int ork_mlp_first_layer_graph(void *stream, const float *gf, int ng, const void *w, int N, float *u) {
+    first_layer_graph_kernel<<<(N + 255) / 256, 256, 0, (hipStream_t)stream>>>(gf, ng, (const bf16 *)w, N, u);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int ork_mlp_first_layer_rows(void *stream, const float *angles, int na, long r0, int rows, int Mp,
+                                        const float *u, const void *wa, const float *bias, int N, void *act) {
+    if (N % 4 || (long)Mp * (N / 4) >= (1L << 32)) return (int)hipErrorInvalidValue;
+    const unsigned grid = g1((long)Mp * (N / 4));
+    if (na == 8)
+        first_layer_rows_kernel<8><<<grid, 256, 0, (hipStream_t)stream>>>(angles, na, r0, rows, Mp, u,
+                                                                         (const bf16 *)wa, bias, N, (bf16 *)act);
+    else
+        first_layer_rows_kernel<0><<<grid, 256, 0, (hipStream_t)stream>>>(angles, na, r0, rows, Mp, u,
+                                                                         (const bf16 *)wa, bias, N, (bf16 *)act);
     ORK_LAUNCH_CHECK();
     return 0;
 }

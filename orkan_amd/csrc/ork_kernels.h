@@ -80,7 +80,7 @@ int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rh
 
 /* launch shape of the density-matrix kernels: which 0 = tile-group kernel, 1 = phase kernel;
  * xcd: keep consecutive groups on one XCD; lds_bytes: LDS per workgroup (caps residency) */
-int ork_dm_set_shape(int which, int xcd, int lds_bytes);
+int ork_dm_set_shape(int which, int flags, int lds_bytes);
 
 /* number of doubles the *_mean / matel launchers need in partials_dev */
 uint64_t ork_partials_len(void);

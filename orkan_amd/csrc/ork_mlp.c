@@ -23,8 +23,9 @@ int ork_mlp_head_forward(void *stream, const void *h, const float *w, float b, i
 int ork_mlp_head_backward(void *stream, const float *dy, const float *w, const float *pre, void *dpre, int M, int K);
 int ork_mlp_colsum(void *stream, const void *a, const float *s, int M, int K, float *out, float *part);
 int ork_mlp_sum(void *stream, const float *v, int n, float *out);
-int ork_mlp_build_features(void *stream, float *X, const float *gf, int ng, const float *angles, int na, long r0,
-                           int rows);
+int ork_mlp_first_layer_graph(void *stream, const float *gf, int ng, const void *w, int N, float *u);
+int ork_mlp_first_layer_rows(void *stream, const float *angles, int na, long r0, int rows, int Mp, const float *u,
+                             const void *wa, const float *bias, int N, void *act);
 int ork_mlp_adam(void *stream, float *w, const float *g, float *m, float *v, long n, int ncols, float lr, float b1,
                  float b2, float eps, float c1, float c2, void *wb, void *wtb, int nrows);
 
@@ -53,6 +54,7 @@ struct orkan_mlp {
     void *tA, *tB;              /* bf16 transposes [hidden][M] */
     float *y, *dy, *sq, *tgt, *scal;
     float *colpart;             /* 64 x hidden partial column sums */
+    float *gf_dev, *u1;         /* screening: graph features [in_dim], their layer-1 share [hidden] */
     long step;
 };
 
@@ -107,6 +109,8 @@ orkan_mlp_t *orkan_mlp_create(int in_dim, int hidden, int nhidden, int max_batch
     q->sq = dmalloc(M * 4, &err); q->tgt = dmalloc(M * 4, &err);
     q->scal = dmalloc(64, &err);
     q->colpart = dmalloc((size_t)64 * H * 4, &err);
+    q->gf_dev = dmalloc((size_t)in_dim * 4, &err);
+    q->u1 = dmalloc(H * 4, &err);
     if (err) {
         orkan_mlp_destroy(q);
         return NULL;
@@ -136,7 +140,7 @@ void orkan_mlp_destroy(orkan_mlp_t *q) {
             if (p[i]) (void)hipFree(p[i]);
     }
     void *p[] = {q->w_out, q->g_out, q->m_out, q->v_out, q->gb_out_d, q->x32, q->x16, q->dpre, q->dpre2,
-                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal, q->colpart};
+                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal, q->colpart, q->gf_dev, q->u1};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i)
         if (p[i]) (void)hipFree(p[i]);
     free(q);
@@ -227,25 +231,31 @@ int orkan_mlp_forward(orkan_mlp_t *q, int batch, const float *x, float *y) {
     return (int)hipStreamSynchronize(st);
 }
 
-/* Screening: rows r = 0..ncand-1 are [gf (ng floats), angles_dev[r] (in_dim - ng floats)];
- * features are assembled on the device chunk by chunk, y_dev receives ncand predictions. */
+/* Screening: rows r = 0..ncand-1 are [gf (ng floats), angles_dev[r] (in_dim - ng floats)].
+ * The graph features' share of layer 1 is computed once (first_layer_graph); each
+ * chunk then builds layer 1's activations from the angles alone and runs the
+ * remaining layers.  y_dev receives ncand predictions. */
 int orkan_mlp_forward_candidates(orkan_mlp_t *q, const float *gf_host, int ng, const float *angles_dev, long ncand,
                                  float *y_dev) {
     if (!q || ng <= 0 || ng >= q->in_dim || ncand < 0) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)ork_dev()->stream;
-    const int na = q->in_dim - ng;
-    int e = (int)hipMemcpyAsync(q->tgt, gf_host, (size_t)ng * 4, hipMemcpyHostToDevice, st);
+    const int na = q->in_dim - ng, H = q->hidden;
+    const layer_t *L0 = &q->L[0];
+    int e = (int)hipMemcpyAsync(q->gf_dev, gf_host, (size_t)ng * 4, hipMemcpyHostToDevice, st);
+    if (!e) e = ork_mlp_first_layer_graph(st, q->gf_dev, ng, L0->wb, H, q->u1);
     for (long r0 = 0; r0 < ncand && !e; r0 += q->max_batch) {
         const int rows = (int)(ncand - r0 < q->max_batch ? ncand - r0 : q->max_batch);
         const int Mp = pad_rows(rows);
-        e = ork_mlp_build_features(st, q->x32, q->tgt, ng, angles_dev, na, r0, rows);
-        if (!e && Mp > rows)
-            e = (int)hipMemsetAsync(q->x32 + (size_t)rows * q->in_dim, 0, (size_t)(Mp - rows) * q->in_dim * 4, st);
-        if (!e) e = ork_mlp_to_bf16(st, q->x32, q->x16, (long)Mp * q->in_dim);
-        if (!e) e = forward_rows(q, Mp, 0);
-        if (!e) e = ork_mlp_head_forward(st, q->act[q->nl - 1], q->w_out, q->b_out, Mp, q->hidden, q->y, NULL, NULL,
+        e = ork_mlp_first_layer_rows(st, angles_dev, na, r0, rows, Mp, q->u1, (const char *)L0->wb + (size_t)ng * H * 2,
+                                     L0->b, H, q->act[0]);
+        const void *in = q->act[0];
+        for (int l = 1; l < q->nl && !e; ++l) {
+            layer_t *L = &q->L[l];
+            e = ork_mlp_gemm(st, 1, in, L->K, L->wtb, L->K, Mp, L->N, L->K, NULL, L->N, L->b, NULL, q->act[l]);
+            in = q->act[l];
+        }
+        if (!e) e = ork_mlp_head_forward(st, q->act[q->nl - 1], q->w_out, q->b_out, Mp, H, y_dev + r0, NULL, NULL,
                                          NULL, rows, 1.0f);
-        if (!e) e = (int)hipMemcpyAsync(y_dev + r0, q->y, (size_t)rows * 4, hipMemcpyDeviceToDevice, st);
     }
     if (!e) e = (int)hipStreamSynchronize(st);
     return e;

@@ -95,8 +95,8 @@ int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count) {
     return (int)hipStreamSynchronize((hipStream_t)d->stream);
 }
 
-int orkan_set_launch_shape(int kernel, int xcd, int lds_bytes) {
-    return ork_dm_set_shape(kernel, xcd, lds_bytes);
+int orkan_set_launch_shape(int kernel, int flags, int lds_bytes) {
+    return ork_dm_set_shape(kernel, flags, lds_bytes);
 }
 
 int orkan_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {

@@ -114,3 +114,36 @@ def test_topk_matches_numpy(n, k):
     assert np.array_equal(y[idx], val)
     assert len(set(idx.tolist())) == k
     lib.orkan_dev_free(p)
+
+
+@pytest.mark.parametrize("ncand,max_batch", [(300, 512), (1000, 256), (129, 128)])
+def test_forward_candidates_matches_forward(ncand, max_batch):
+    """Screening forward (graph share of layer 1 computed once, angles per row,
+    candidates in chunks of max_batch with a ragged last chunk) against the CPU
+    oracle and against the plain forward on the assembled rows."""
+    lib = orkan_amd.load(5)
+    rng = np.random.default_rng(ncand)
+    W, b, wo, bo = _params(rng)
+    m = surrogate.Mlp(lib, max_batch=max_batch)
+    m.set_weights(W, b, wo, bo)
+    gf = surrogate.graph_features(rng.uniform(0, 1, 60))
+    ang = rng.uniform(0, 1, (ncand, 8)).astype(np.float32)
+    x = np.concatenate([np.broadcast_to(gf, (ncand, 120)), ang], 1).astype(np.float32)
+    params = ([w.astype(np.float64) for w in W], [v.astype(np.float64) for v in b], wo.astype(np.float64), bo)
+    y_ref, _, _ = mlp_np.forward(params, x.astype(np.float64), exact=False)
+    dang = upload(lib, ang)
+    # one guard float past the outputs: the forward must not write beyond ncand
+    ydev = lib.orkan_dev_alloc((ncand * 4 + 4 + 7) // 8)
+    guard = np.full(ncand + 1, 12345.0, np.float32)
+    assert lib.orkan_copy_to_device(ydev, guard.ctypes.data, guard.nbytes) == 0
+    m.forward_candidates(gf, dang, ncand, ydev)
+    y = np.empty(ncand + 1, np.float32)
+    assert lib.orkan_copy_to_host(y.ctypes.data, ydev, y.nbytes) == 0
+    assert y[ncand] == 12345.0
+    y = y[:ncand]
+    scale = max(1.0, np.abs(y_ref).max())
+    assert np.abs(y - y_ref).max() <= 2e-3 * scale
+    assert np.abs(y - m.forward(x)).max() <= 2e-3 * scale
+    lib.orkan_dev_free(dang)
+    lib.orkan_dev_free(ydev)
+    m.close()
