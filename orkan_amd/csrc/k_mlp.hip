@@ -141,26 +141,28 @@ __global__ __launch_bounds__(GT2) void gemm_nt_big_kernel(const bf16 *__restrict
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves: 128 rows x 64 columns each
     const long m0 = (long)blockIdx.y * BM2, n0 = (long)blockIdx.x * BN2;
-    // staging: each thread moves 32 consecutive K elements of one row of A and of B per K step
-    const int lr = tid >> 1, lk = (tid & 1) * 32;
+    // staging: 8 lanes cover the 128 B of one row's K step, a wave 8 rows (fully used cache
+    // lines); each thread moves 16 B of rows lr, lr + 64, lr + 128, lr + 192 of A and of B
+    const int lr = tid >> 3, lk = (tid & 7) * 8;
     const bf16 *ga = A + (m0 + lr) * (long)lda + lk;
     const bf16 *gb = B + (n0 + lr) * (long)ldb + lk;
+    const long sa = 64 * (long)lda, sb = 64 * (long)ldb;
     uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define BIG_GLOAD(k0)                                                                   \
-    do {                                                                                \
-        ra0 = *(const uint4 *)(ga + (k0));      ra1 = *(const uint4 *)(ga + (k0) + 8);  \
-        ra2 = *(const uint4 *)(ga + (k0) + 16); ra3 = *(const uint4 *)(ga + (k0) + 24); \
-        rb0 = *(const uint4 *)(gb + (k0));      rb1 = *(const uint4 *)(gb + (k0) + 8);  \
-        rb2 = *(const uint4 *)(gb + (k0) + 16); rb3 = *(const uint4 *)(gb + (k0) + 24); \
+#define BIG_GLOAD(k0)                                                                            \
+    do {                                                                                         \
+        ra0 = *(const uint4 *)(ga + (k0));          ra1 = *(const uint4 *)(ga + sa + (k0));      \
+        ra2 = *(const uint4 *)(ga + 2 * sa + (k0)); ra3 = *(const uint4 *)(ga + 3 * sa + (k0));  \
+        rb0 = *(const uint4 *)(gb + (k0));          rb1 = *(const uint4 *)(gb + sb + (k0));      \
+        rb2 = *(const uint4 *)(gb + 2 * sb + (k0)); rb3 = *(const uint4 *)(gb + 3 * sb + (k0));  \
     } while (0)
 #define BIG_SSTORE(buf)                                                                  \
     do {                                                                                 \
         bf16 *pa = As + (buf) * BM2 * KP2 + lr * KP2 + lk;                               \
         bf16 *pb = Bs + (buf) * BN2 * KP2 + lr * KP2 + lk;                               \
-        *(uint4 *)(pa) = ra0; *(uint4 *)(pa + 8) = ra1;                                  \
-        *(uint4 *)(pa + 16) = ra2; *(uint4 *)(pa + 24) = ra3;                            \
-        *(uint4 *)(pb) = rb0; *(uint4 *)(pb + 8) = rb1;                                  \
-        *(uint4 *)(pb + 16) = rb2; *(uint4 *)(pb + 24) = rb3;                            \
+        *(uint4 *)(pa) = ra0; *(uint4 *)(pa + 64 * KP2) = ra1;                           \
+        *(uint4 *)(pa + 128 * KP2) = ra2; *(uint4 *)(pa + 192 * KP2) = ra3;              \
+        *(uint4 *)(pb) = rb0; *(uint4 *)(pb + 64 * KP2) = rb1;                           \
+        *(uint4 *)(pb + 128 * KP2) = rb2; *(uint4 *)(pb + 192 * KP2) = rb3;              \
     } while (0)
     f32x16 acc[4][2];
 #pragma unroll

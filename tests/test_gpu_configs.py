@@ -204,12 +204,13 @@ def test_fp32_density_matrix_drift(dev, kind):
 
 
 # ---- streamed tiled superoperator vs the LDS tile-group kernel ---------------------------------
-@pytest.mark.parametrize("lt,n", [(5, 8), (5, 9), (3, 7)])
+@pytest.mark.parametrize("kind,lt,n", [(MIXED_TILED, 5, 8), (MIXED_TILED, 5, 9), (MIXED_TILED, 3, 7),
+                                       (MIXED_PACKED, 5, 6), (MIXED_PACKED, 5, 9)])
 @pytest.mark.parametrize("fp32", [False, True])
-def test_streamed_superop_matches_tile_groups(lt, n, fp32):
-    """The streamed kernels (target inside a tile; target above it, with and
-    without a mirrored tile, and diagonal tile quads) against the tile-group
-    kernel.  Both form the same products in the same order; where a stored
+def test_streamed_superop_matches_tile_groups(kind, lt, n, fp32):
+    """The streamed kernels (tiled: target inside a tile; target above it, with
+    and without a mirrored tile, and diagonal tile quads; packed: direct,
+    mirrored and diagonal quads) against the tile-group kernel.  Both form the same products in the same order; where a stored
     entry is the mirror of an upper-triangle entry, the tile-group kernel may
     write it as the conjugate of that entry's value instead, which differs by
     rounding only.  One gate from the same input agrees to 1e-15 (FP64) and a
@@ -217,14 +218,14 @@ def test_streamed_superop_matches_tile_groups(lt, n, fp32):
     eng = engines.DeviceEngine(orkan_amd.lib_path(lt), lt)
     api = eng.api
     g = wl.rng(17 + n)
-    rho = to_layout(wl.low_rank_rho(n, 16, g), MIXED_TILED, lt)
+    rho = to_layout(wl.low_rank_rho(n, 16, g), kind, lt)
     S = wl.superop(wl.kraus_amp_damp(0.05))
     ops = [(lambda st, q: eng.gate(st, "h", q)), (lambda st, q: eng.gate(st, "ry", q, 0.3 + 0.1 * q)),
            (lambda st, q: eng.channel(st, S, q))]
 
     def run(on, seq):
         assert api.orkan_set_launch_shape(8, 0, on) == 0
-        st = eng.new(MIXED_TILED, n, rho)
+        st = eng.new(kind, n, rho)
         if fp32:
             st32 = api.orkan_state_convert(C.byref(st), 1)
             eng.free(st)
