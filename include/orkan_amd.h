@@ -69,7 +69,7 @@ int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);
  * for a graph's cost diagonal C (2^n doubles).  All evaluations of one call
  * run in a single persistent kernel launch replayed from a hipGraph; states of
  * up to 2^LT amplitudes stay in LDS, larger ones (n <= 2 LT) stream through HBM
- * in two LDS-tiled sweeps per layer.
+ * in p + 1 LDS-tiled sweeps.
  * ---------------------------------------------------------------------------------------------- */
 typedef struct orkan_qaoa orkan_qaoa_t;
 

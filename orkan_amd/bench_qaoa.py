@@ -59,15 +59,16 @@ def run(lib, root, reps=5, cpu_budget_s=15.0, count=4096):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     evals_s = len(inst) / t
-    # streamed bytes actually moved by the two-sweep schedule (complex128 in + out per sweep,
-    # the cost diagonal once per layer; LDS-resident states move nothing)
+    # bytes the p + 1 sweeps of an HBM-resident state move, per amplitude: the first sweep
+    # reads the cost diagonal and writes the state (8 + 16), the p - 1 middle ones read and
+    # write the state and read the diagonal (40), the last reads both (24): 40 p + 8.
+    # LDS-resident states move nothing.
     moved = 0
     for i in inst:
-        N = 1 << i["n"]
         if i["n"] > cfg["lt"]:
-            moved += i["p"] * (2 * 2 * 16 * N + 8 * N)
+            moved += (40 * i["p"] + 8) * (1 << i["n"])
     res = {"instances": len(inst), "evals_per_s": round(evals_s, 1), "batch_ms": round(t * 1e3, 3),
-           "autotuned": cfg, "hbm_GBs_two_sweep": round(moved / t / 1e9, 1)}
+           "autotuned": cfg, "hbm_GBs_sweeps": round(moved / t / 1e9, 1)}
     # CPU sample through the reference library: same evaluations, per-instance call sequence
     ref_path = os.path.join(root, "oracle", "_ref", "liborkan_ref_t5.so")
     kind = "reference"

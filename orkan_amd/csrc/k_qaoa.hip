@@ -6,17 +6,19 @@
 //   mean(C)
 // (src/state/state_pure.c:30-43, src/circ/circ_pure.c:21-34,
 //  src/gate/gate_pure.c:221-234, src/meas/meas_pure.c:20-30), with the same
-// per-amplitude formulas and the same qubit order, so each instance matches a
-// per-instance run of those calls to rounding.
+// per-amplitude formulas, so each instance matches a per-instance run of those
+// calls to rounding (the rx of one layer act on distinct qubits and commute;
+// states in HBM apply them in another order than 0..n-1).
 //
 // Execution: every workgroup of the persistent grid pulls evaluations from an
 // atomic counter (largest first) and runs each one start to finish, so no
 // workgroup ever waits for another.  A state of n <= LT qubits lives in LDS for
-// the whole circuit.  A larger state lives in HBM and each layer is two passes
-// of LDS tiles of 2^LT amplitudes: pass A (contiguous tiles) applies the
-// exp_diag phases and the rx of qubits 0..LT-1, pass B (tiles gathered at
-// stride 2^LT) applies the rx of qubits LT..n-1; the last pass also reduces
-// sum_x C(x) |psi_x|^2.  That is 2 sweeps per layer instead of n+1.
+// the whole circuit.  A larger state (n <= 2 LT) lives in HBM and is swept in
+// LDS tiles of 2^LT amplitudes, alternating contiguous tiles and tiles gathered
+// at stride 2^LT; each sweep finishes the previous layer's rx on the qubits the
+// previous sweep could not reach and runs the next layer's exp_diag and rx on
+// every qubit it holds: p + 1 sweeps for p layers instead of p (n + 1) gate
+// passes; the last sweep reduces sum_x C(x) |psi_x|^2.
 #include "k_common.hpp"
 
 namespace {
@@ -154,41 +156,50 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
                 acc += C[x] * (v.x * v.x + v.y * v.y);
             }
         } else {
-            // ---- state in HBM, two LDS-tiled passes per layer ----
-            const int hb = n - LT;                 // qubits above the tile
-            const u64 nchunk = (u64)1 << hb;
-            const int ls = 2 * LT - n;             // log2 of the contiguous run in pass B
+            // ---- state in HBM: p + 1 LDS-tiled passes ----
+            // Pass k alternates between A tiles (contiguous: qubits 0..LT-1) and B tiles (all
+            // 2^hb high indices x a run of 2^ls low indices: qubits 0..ls-1 and LT..n-1); both
+            // hold tile bits ls..LT-1 that the other does not (A: qubits ls..LT-1, B: LT..n-1).
+            // Pass k first finishes layer k-1 (rx on tile bits ls..LT-1, the qubits the previous
+            // pass could not reach), then starts layer k (phase, rx on every tile bit), so one
+            // pass per layer reads and writes the state; pass p reduces the mean instead.
+            const int hb = n - LT;                 // qubits above the A tile
+            const int ls = 2 * LT - n;             // log2 of the low run of a B tile
             const int S = 1 << ls;
-            for (int l = 0; l < ev.p; ++l) {
-                const double g = ang[3 * l], c = ang[3 * l + 1], s = ang[3 * l + 2];
-                const bool last = (l == ev.p - 1);
-                // pass A: contiguous tiles, phases + rx on qubits 0..LT-1
+            const u64 nchunk = (u64)1 << hb;
+            for (int k = 0; k <= ev.p; ++k) {
+                const bool btile = k & 1;
+                const bool last = (k == ev.p);
+                const double g = last ? 0.0 : ang[3 * k], c = last ? 0.0 : ang[3 * k + 1],
+                             s = last ? 0.0 : ang[3 * k + 2];
+                const double cp = k ? ang[3 * (k - 1) + 1] : 0.0, sp = k ? ang[3 * (k - 1) + 2] : 0.0;
                 for (u64 ch = 0; ch < nchunk; ++ch) {
-                    const u64 base = ch << LT;
-                    for (int x = threadIdx.x; x < T; x += THREADS) {
-                        const cd v = (l == 0) ? cmk(amp, 0.0) : slot[base + x];
-                        tile[pad(x)] = phase(v, C[base + x], g);
-                    }
-                    __syncthreads();
-                    lds_rx<THREADS>(tile, LT, 0, LT, c, s);
-                    for (int x = threadIdx.x; x < T; x += THREADS) slot[base + x] = tile[pad(x)];
-                    __syncthreads();
-                }
-                // pass B: tile = all 2^hb high indices x a run of S low indices
-                for (u64 ch = 0; ch < ((u64)1 << (LT - ls)); ++ch) {
                     const u64 lo0 = ch << ls;
-                    for (int x = threadIdx.x; x < T; x += THREADS) {
-                        const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
-                        tile[pad(x)] = slot[(hi << LT) | lo];
+                    // global index of tile element x
+#define QIDX(x) (btile ? ((((u64)(x) >> ls) << LT) | (lo0 + (u64)((x) & (S - 1)))) : ((ch << LT) + (u64)(x)))
+                    if (k == 0) {
+                        for (int x = threadIdx.x; x < T; x += THREADS) tile[pad(x)] = phase(cmk(amp, 0.0), C[QIDX(x)], g);
+                        __syncthreads();
+                    } else {
+                        for (int x = threadIdx.x; x < T; x += THREADS) tile[pad(x)] = slot[QIDX(x)];
+                        __syncthreads();
+                        lds_rx<THREADS>(tile, LT, ls, LT, cp, sp);
+                        if (!last) {
+                            for (int x = threadIdx.x; x < T; x += THREADS)
+                                tile[pad(x)] = phase(tile[pad(x)], C[QIDX(x)], g);
+                            __syncthreads();
+                        }
                     }
-                    __syncthreads();
-                    lds_rx<THREADS>(tile, LT, ls, ls + hb, c, s);
-                    for (int x = threadIdx.x; x < T; x += THREADS) {
-                        const u64 hi = (u64)(x >> ls), lo = lo0 + (u64)(x & (S - 1));
-                        const cd v = tile[pad(x)];
-                        if (last) acc += C[(hi << LT) | lo] * (v.x * v.x + v.y * v.y);
-                        else slot[(hi << LT) | lo] = v;
+                    if (!last) {
+                        lds_rx<THREADS>(tile, LT, 0, LT, c, s);
+                        for (int x = threadIdx.x; x < T; x += THREADS) slot[QIDX(x)] = tile[pad(x)];
+                    } else {
+                        for (int x = threadIdx.x; x < T; x += THREADS) {
+                            const cd v = tile[pad(x)];
+                            acc += C[QIDX(x)] * (v.x * v.x + v.y * v.y);
+                        }
                     }
+#undef QIDX
                     __syncthreads();
                 }
             }
