@@ -26,6 +26,27 @@ extern "C" {
 state_t orkan_state_convert(const state_t *state, int to_fp32);
 
 /* ------------------------------------------------------------------------------------------------
+ * Layout conversion and state files.  No counterpart in the reference, which has neither a
+ * conversion nor a file format but reserves QS_ERR_FILE / QS_ERR_FORMAT for one
+ * (include/q_types.h:35-36).
+ * ---------------------------------------------------------------------------------------------- */
+/* *dst = new state (device memory, the source's precision) holding the logical density matrix of
+ * src in `layout` (MIXED_PACKED or MIXED_TILED); a PURE source gives |psi><psi|.  dst's previous
+ * contents are overwritten, not freed.  QS_ERR_NULL, QS_ERR_TYPE (pure or unknown destination,
+ * unknown source type), QS_ERR_OOM. */
+qs_error_t orkan_state_relayout(const state_t *src, state_t *dst, state_type_t layout);
+/* File = 64-byte header ("ORKAN-ST", version 1, type incl. ORKAN_FP32, qubits, LOG_TILE_DIM,
+ * stored length, element bytes, byte-order mark 0x01020304; little-endian), then the storage
+ * array exactly as the state holds it.  save: QS_ERR_NULL, QS_ERR_TYPE, QS_ERR_FILE (open, write
+ * or device copy failed), QS_ERR_OOM.  load: frees the state's old data (state_free) and
+ * replaces type, qubits and data; QS_ERR_FILE (open or read failed), QS_ERR_FORMAT (bad magic,
+ * version, byte order or type, length not matching type and qubits, tiled file of another
+ * LOG_TILE_DIM, payload not exactly the stated length), QS_ERR_OOM; on error the state is left
+ * unchanged. */
+qs_error_t orkan_state_save(const state_t *state, const char *path);
+qs_error_t orkan_state_load(state_t *state, const char *path);
+
+/* ------------------------------------------------------------------------------------------------
  * Bulk transfer of the storage array (state_len elements in the state's own
  * precision: cplx_t, or float pairs for ORKAN_FP32).  Complements state_get /
  * state_set (reference include/state.h:128,141), which move one element.

@@ -9,6 +9,13 @@
  *   channel_packed_1q (lower / upper-conj / diagonal block) src/channel/channel_packed.c:48-120
  *   channel_tiled_1q  (within-tile; cross-tile a, b, c)    src/channel/channel_tiled.c:52-185
  * The superoperator step itself is op1_apply(OP_SOP) in orc_ops.h.
+ *
+ * channel_2q has no counterpart in the reference (channel_1q rejects n_qubits != 1,
+ * src/channel/channel.c:71; kraus_to_superop already builds n-qubit superoperators,
+ * src/channel/channel.c:24-47): every 4x4 block of the qubit pair, read and written through
+ * the Hermitian state_get / state_set, gets out = S vec(block).  Parity unpinned by the
+ * reference; pinned by tests/test_oracle_math.py against sum_a K_a rho K_a^dagger on dense
+ * matrices.
  */
 #include "channel.h"
 #include "orc_internal.h"
@@ -66,4 +73,16 @@ void channel_1q(state_t *state, const superop_t *sop, const qubit_t target) {
     default:
         CV(0, "channel_1q: unknown state type");
     }
+}
+
+void channel_2q(state_t *state, const superop_t *sop, const qubit_t q1, const qubit_t q2) {
+    CV(state && state->data, "channel_2q: null state or data pointer");
+    CV(q1 < state->qubits && q2 < state->qubits, "channel_2q: qubit out of range");
+    CV(q1 != q2, "channel_2q: qubits must differ");
+    CV(sop && sop->data, "channel_2q: null superoperator or data pointer");
+    CV(sop->n_qubits == 2, "channel_2q: superoperator is not 2-local");
+    CV(state->type == PURE || state->type == MIXED_PACKED || state->type == MIXED_TILED,
+       "channel_2q: unknown state type");
+    CV(state->type != PURE, "channel_2q: not available for pure states");
+    orc_dm_sop2(state, q1, q2, sop->data);
 }

@@ -297,6 +297,23 @@ static void apply_mono(state_t *st, const mono_t *m) {
     else dm_groups(st, m->qs, m->k, grp_mono, m);
 }
 
+/* 2-qubit superoperator on every 4x4 block (channel_2q): local index bit 0 <-> q1, bit 1 <-> q2;
+ * S row-major on vec(block), out[(a,b)] = sum_(c,d) S[(a,b),(c,d)] in[(c,d)] */
+static void grp_sop2(const cplx_t *in, cplx_t *out, int K, const void *ctx) {
+    const cplx_t *S = ctx;
+    (void)K;
+    for (int i = 0; i < 16; ++i) {
+        cplx_t acc = 0.0;
+        for (int j = 0; j < 16; ++j) acc += S[i * 16 + j] * in[j];
+        out[i] = acc;
+    }
+}
+
+void orc_dm_sop2(state_t *st, unsigned q1, unsigned q2, const cplx_t *S) {
+    const unsigned qs[2] = {q1, q2};
+    dm_groups(st, qs, 2, grp_sop2, S);
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Public gates
  * ---------------------------------------------------------------------------------------------- */

@@ -23,6 +23,9 @@ EXTRA = {
     "orkan_state_upload": (C.c_int, [_ffi.PS, C.c_void_p]),
     "orkan_state_download": (C.c_int, [_ffi.PS, C.c_void_p]),
     "orkan_state_convert": (_ffi.StateT, [_ffi.PS, C.c_int]),
+    "orkan_state_relayout": (C.c_int, [_ffi.PS, _ffi.PS, C.c_int]),
+    "orkan_state_save": (C.c_int, [_ffi.PS, C.c_char_p]),
+    "orkan_state_load": (C.c_int, [_ffi.PS, C.c_char_p]),
     "orkan_dev_alloc": (C.c_void_p, [C.c_uint64]),
     "orkan_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "orkan_dev_free": (None, [C.c_void_p]),

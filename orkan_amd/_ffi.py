@@ -65,6 +65,7 @@ API = {
     "ccx": (None, [PS, Q, Q, Q]),
     "kraus_to_superop": (SuperopT, [C.POINTER(KrausT)]),
     "channel_1q": (None, [PS, C.POINTER(SuperopT), Q]),
+    "channel_2q": (None, [PS, C.POINTER(SuperopT), Q, Q]),
     "mean": (D, [PS, DP]),
     "matel_diag": (Cplx, [PS, PS, DP]),
     "exp_diag": (None, [PS, DP, D]),

@@ -52,6 +52,7 @@ struct DmOp {
     int pinv[8];    // MONO: local index a' <- pinv[a']
     int ph[8];      // MONO: U|a> = i^ph[a] |perm a>
     cd m[64];       // SOP: 4x4 row-major; DENSE: 2^k x 2^k row-major
+    const cd *s2;   // SOP2: 16x16 row-major superoperator in device memory
 };
 
 __device__ __forceinline__ u64 spread_bits(int a, const int *pos, int k) {
@@ -184,6 +185,32 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 lds[x0 * LDS_STRIDE + y1] = o[1];
                 lds[x1 * LDS_STRIDE + y0] = o[2];
                 lds[x1 * LDS_STRIDE + y1] = o[3];
+            }
+            __syncthreads();
+        } else if constexpr (OP == ORK_DM_SOP2) {
+            // 2-qubit superoperator on each 4x4 block: out[(a,b)] = sum_(c,d) S[(a,b),(c,d)] in[(c,d)]
+            const int LW2 = LW - 2;
+            for (int b = tid; b < (total >> 4); b += DM_THREADS) {
+                const int x0 = (int)ins0k((u64)(b >> LW2), op.srt, 2);
+                const int y0 = (int)ins0k((u64)(b & ((1 << LW2) - 1)), op.srt, 2);
+                int xo[4], yo[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    xo[a] = (x0 | (int)spread_bits(a, op.lbit, 2)) * LDS_STRIDE;
+                    yo[a] = y0 | (int)spread_bits(a, op.lbit, 2);
+                }
+                cd v[16];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) v[a * 4 + c] = lds[xo[a] + yo[c]];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    cd o = cmk(0.0, 0.0);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) cfma(o, op.s2[i * 16 + j], v[j]);
+                    lds[xo[i >> 2] + yo[i & 3]] = o;
+                }
             }
             __syncthreads();
         } else if constexpr (OP == ORK_DM_DENSE) {
@@ -586,6 +613,7 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
     case ORK_DM_MONO: dm_group_kernel<S, LAYOUT, ORK_DM_MONO><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
     case ORK_DM_DENSE: dm_group_kernel<S, LAYOUT, ORK_DM_DENSE><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
+    case ORK_DM_SOP2: dm_group_kernel<S, LAYOUT, ORK_DM_SOP2><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
     default: return (int)hipErrorInvalidValue;
     }
     ORK_LAUNCH_CHECK();
@@ -728,6 +756,8 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     const int nm = (opk == ORK_DM_SOP) ? 16 : (opk == ORK_DM_DENSE ? (1 << (2 * k)) : 0);
     for (int i = 0; i < 64; ++i) op.m[i] = make_double2(0.0, 0.0);
     for (int i = 0; i < nm; ++i) op.m[i] = make_double2(mat[2 * i], mat[2 * i + 1]);
+    op.s2 = opk == ORK_DM_SOP2 ? (const cd *)mat : nullptr;
+    if (opk == ORK_DM_SOP2 && k != 2) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
     if (layout == ORK_PACKED && opk == ORK_DM_SOP && g_stream_on && n >= 6)
         return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, m->qs[0], op)

@@ -5,7 +5,10 @@
  * frees, exactly as the reference (src/channel/channel.c:21-50):
  *   S[(i,j),(k,l)] = sum_a K_a[i,k] conj(K_a[j,l]),  row-major, 4^n x 4^n.
  * channel_1q validates like src/channel/channel.c:67-85 and applies the 4x4
- * superoperator on the device with the tile-group kernel.
+ * superoperator on the device (streamed kernels, tile groups where they do not
+ * apply).  channel_2q is an extension: the reference's kraus_to_superop already
+ * builds 2-qubit superoperators (src/channel/channel.c:24-47) but channel_1q
+ * accepts only 1-qubit ones (src/channel/channel.c:71).
  */
 #include "ork_host.h"
 #include "orkan_backend.h"
@@ -66,4 +69,27 @@ void channel_1q(state_t *state, const superop_t *sop, qubit_t target) {
         channel_tiled_1q(state, sop->data, target);
         break;
     }
+}
+
+/* rho -> S(rho) on the qubit pair (q1, q2): S is 16 x 16, row-major on vec of the 4 x 4 block
+ * whose local index is a = x_q1 + 2 x_q2 (q1 is the low bit). */
+void channel_2q(state_t *state, const superop_t *sop, qubit_t q1, qubit_t q2) {
+    CV(state && state->data, "channel_2q: null state or data pointer");
+    CV(q1 < state->qubits && q2 < state->qubits, "channel_2q: qubit out of range");
+    CV(q1 != q2, "channel_2q: qubits must differ");
+    CV(sop && sop->data, "channel_2q: null superoperator or data pointer");
+    CV(sop->n_qubits == 2, "channel_2q: superoperator is not 2-local");
+    CV(ork_type_ok(state), "channel_2q: unknown state type");
+    CV(ork_layout(state) != PURE, "channel_2q: not available for pure states");
+    ork_mono_t m;
+    memset(&m, 0, sizeof(m));
+    m.k = 2;
+    m.qs[0] = q1;
+    m.qs[1] = q2;
+    ork_lock();
+    const double *S = ork_dev_doubles((const double *)sop->data, 2 * 256);
+    const int e = ork_dm_group(ork_dev()->stream, ork_prec(state), ork_layout(state), LOG_TILE_DIM, state->data,
+                               state->qubits, ORK_DM_SOP2, &m, S);
+    ork_unlock();
+    ORK_HIP("channel_2q", e);
 }

@@ -43,7 +43,9 @@ enum {
 enum {
     ORK_DM_SOP = 0,    /* 1 qubit, 4x4 superoperator (row-major on (r00, r01, r10, r11)) */
     ORK_DM_MONO = 1,   /* k <= 3 qubits, basis permutation with phases in {1, i, -1, -i} */
-    ORK_DM_DENSE = 2   /* k <= 3 qubits, dense 2^k x 2^k unitary, rho -> U rho U^dagger */
+    ORK_DM_DENSE = 2,  /* k <= 3 qubits, dense 2^k x 2^k unitary, rho -> U rho U^dagger */
+    ORK_DM_SOP2 = 3    /* 2 qubits, 16x16 superoperator on the 4x4 block (row-major on vec(block),
+                        * local index bit j <-> qs[j]); `mat` is then a DEVICE pointer to 256 complex */
 };
 
 /* Monomial (permutation-with-phase) action of a k-qubit gate:
@@ -84,6 +86,11 @@ int ork_dm_set_shape(int which, int flags, int lds_bytes);
 
 /* number of doubles the *_mean / matel launchers need in partials_dev */
 uint64_t ork_partials_len(void);
+
+/* dst (packed or tiled, len stored entries) <- the logical density matrix of src (pure: |psi><psi|,
+ * packed or tiled), same precision */
+int ork_relayout(void *stream, int prec, const void *src, int src_layout, void *dst, int dst_layout, int n,
+                 int log_tile, uint64_t len);
 
 /* complex128 <-> complex64 conversion of a whole storage array */
 int ork_convert(void *stream, const void *src_dev, int src_prec, void *dst_dev, int dst_prec, uint64_t len);

@@ -324,3 +324,144 @@ state_t orkan_state_convert(const state_t *state, int to_fp32) {
             ork_convert(ork_dev()->stream, state->data, ork_prec(state), out.data, ork_prec(&out), len));
     return out;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Extensions without a reference counterpart: layout conversion on the device and a state file
+ * format that uses the reference's reserved QS_ERR_FILE / QS_ERR_FORMAT codes
+ * (include/q_types.h:35-36).
+ * ---------------------------------------------------------------------------------------------- */
+
+qs_error_t orkan_state_relayout(const state_t *src, state_t *dst, state_type_t layout) {
+    if (!src || !src->data || !dst) return QS_ERR_NULL;
+    if (!ork_type_ok(src)) return QS_ERR_TYPE;
+    if (layout != MIXED_PACKED && layout != MIXED_TILED) return QS_ERR_TYPE;
+    state_t out;
+    out.type = (state_type_t)((int)layout | ((int)src->type & ORKAN_FP32));
+    out.qubits = src->qubits;
+    const idx_t len = ork_len((int)layout, src->qubits);
+    out.data = dev_zeros((size_t)len * ork_elem_bytes(&out));
+    if (!out.data) return QS_ERR_OOM;
+    ork_dev_t *d = ork_dev();
+    ORK_HIP("orkan_state_relayout", ork_relayout(d->stream, ork_prec(src), src->data, ork_layout(src), out.data,
+                                                 (int)layout, src->qubits, LOG_TILE_DIM, len));
+    ORK_HIP("orkan_state_relayout", hipStreamSynchronize((hipStream_t)d->stream));
+    *dst = out;
+    return QS_OK;
+}
+
+/* File: a 64-byte little-endian header, then the storage array as the state holds it. */
+#define ORK_FILE_MAGIC "ORKAN-ST"
+#define ORK_FILE_VERSION 1u
+#define ORK_FILE_CHUNK ((size_t)64 << 20)
+
+typedef struct {
+    char magic[8];
+    uint32_t version;
+    uint32_t type;        /* state->type, ORKAN_FP32 flag included */
+    uint32_t qubits;
+    uint32_t log_tile;    /* LOG_TILE_DIM of the writer (tiled layout only must match) */
+    uint64_t length;      /* stored elements */
+    uint32_t elem_bytes;  /* 16 (complex128) or 8 (complex64) */
+    uint32_t byte_order;  /* 0x01020304 as written by the host */
+    uint8_t reserved[24];
+} ork_file_header_t;
+
+_Static_assert(sizeof(ork_file_header_t) == 64, "state file header is 64 bytes");
+
+qs_error_t orkan_state_save(const state_t *state, const char *path) {
+    if (!state || !state->data || !path) return QS_ERR_NULL;
+    if (!ork_type_ok(state)) return QS_ERR_TYPE;
+    ork_file_header_t h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, ORK_FILE_MAGIC, 8);
+    h.version = ORK_FILE_VERSION;
+    h.type = (uint32_t)state->type;
+    h.qubits = state->qubits;
+    h.log_tile = LOG_TILE_DIM;
+    h.length = state_len(state);
+    h.elem_bytes = (uint32_t)ork_elem_bytes(state);
+    h.byte_order = 0x01020304u;
+    FILE *f = fopen(path, "wb");
+    if (!f) return QS_ERR_FILE;
+    qs_error_t rc = fwrite(&h, sizeof(h), 1, f) == 1 ? QS_OK : QS_ERR_FILE;
+    const size_t total = (size_t)h.length * h.elem_bytes;
+    char *buf = rc == QS_OK ? malloc(total < ORK_FILE_CHUNK ? (total ? total : 1) : ORK_FILE_CHUNK) : NULL;
+    if (rc == QS_OK && !buf) rc = QS_ERR_OOM;
+    ork_dev_t *d = ork_dev();
+    for (size_t off = 0; rc == QS_OK && off < total; off += ORK_FILE_CHUNK) {
+        const size_t n = total - off < ORK_FILE_CHUNK ? total - off : ORK_FILE_CHUNK;
+        if (hipMemcpyAsync(buf, (const char *)state->data + off, n, hipMemcpyDeviceToHost,
+                           (hipStream_t)d->stream) != hipSuccess ||
+            hipStreamSynchronize((hipStream_t)d->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = QS_ERR_FILE;
+            break;
+        }
+        if (fwrite(buf, 1, n, f) != n) rc = QS_ERR_FILE;
+    }
+    free(buf);
+    if (fclose(f) != 0 && rc == QS_OK) rc = QS_ERR_FILE;
+    return rc;
+}
+
+qs_error_t orkan_state_load(state_t *state, const char *path) {
+    if (!state || !path) return QS_ERR_NULL;
+    FILE *f = fopen(path, "rb");
+    if (!f) return QS_ERR_FILE;
+    ork_file_header_t h;
+    qs_error_t rc = QS_OK;
+    if (fread(&h, sizeof(h), 1, f) != 1) rc = ferror(f) ? QS_ERR_FILE : QS_ERR_FORMAT;
+    state_t probe;
+    if (rc == QS_OK) {
+        probe.type = (state_type_t)h.type;
+        probe.qubits = (qubit_t)h.qubits;
+        probe.data = NULL;
+        const int layout = (int)h.type & 0xff;
+        if (memcmp(h.magic, ORK_FILE_MAGIC, 8) != 0 || h.version != ORK_FILE_VERSION || h.byte_order != 0x01020304u ||
+            h.qubits > 63 || !ork_type_ok(&probe) || h.elem_bytes != ork_elem_bytes(&probe) ||
+            h.length != ork_len(layout, (qubit_t)h.qubits) || (layout == MIXED_TILED && h.log_tile != LOG_TILE_DIM))
+            rc = QS_ERR_FORMAT;
+    }
+    const size_t total = rc == QS_OK ? (size_t)h.length * h.elem_bytes : 0;
+    void *dev = NULL;
+    if (rc == QS_OK) {
+        /* the payload must be exactly the header's length */
+        const long here = ftell(f);
+        if (here < 0 || fseek(f, 0, SEEK_END) != 0) rc = QS_ERR_FILE;
+        else {
+            const long end = ftell(f);
+            if (end < 0 || fseek(f, here, SEEK_SET) != 0) rc = QS_ERR_FILE;
+            else if ((size_t)(end - here) != total) rc = QS_ERR_FORMAT;
+        }
+    }
+    if (rc == QS_OK) {
+        dev = dev_zeros(total);
+        if (!dev) rc = QS_ERR_OOM;
+    }
+    char *buf = rc == QS_OK ? malloc(total < ORK_FILE_CHUNK ? (total ? total : 1) : ORK_FILE_CHUNK) : NULL;
+    if (rc == QS_OK && !buf) rc = QS_ERR_OOM;
+    ork_dev_t *d = rc == QS_OK ? ork_dev() : NULL;   /* no device use for a rejected file */
+    for (size_t off = 0; rc == QS_OK && off < total; off += ORK_FILE_CHUNK) {
+        const size_t n = total - off < ORK_FILE_CHUNK ? total - off : ORK_FILE_CHUNK;
+        if (fread(buf, 1, n, f) != n) {
+            rc = ferror(f) ? QS_ERR_FILE : QS_ERR_FORMAT;
+            break;
+        }
+        if (hipMemcpyAsync((char *)dev + off, buf, n, hipMemcpyHostToDevice, (hipStream_t)d->stream) != hipSuccess ||
+            hipStreamSynchronize((hipStream_t)d->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = QS_ERR_FILE;
+        }
+    }
+    free(buf);
+    fclose(f);
+    if (rc != QS_OK) {
+        if (dev) dev_release(dev);
+        return rc;
+    }
+    state_free(state);
+    state->type = (state_type_t)h.type;
+    state->qubits = (qubit_t)h.qubits;
+    state->data = dev;
+    return QS_OK;
+}

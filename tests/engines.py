@@ -62,6 +62,11 @@ class Engine:
         so = _ffi.SuperopT(1, sop.ctypes.data)
         self.api.channel_1q(C.byref(st), C.byref(so), q)
 
+    def channel2(self, st, sop, q1, q2):
+        sop = np.ascontiguousarray(sop, dtype=np.complex128).reshape(256)
+        so = _ffi.SuperopT(2, sop.ctypes.data)
+        self.api.channel_2q(C.byref(st), C.byref(so), q1, q2)
+
     def mean(self, st, obs):
         obs = np.ascontiguousarray(obs, dtype=np.float64)
         return float(self.api.mean(C.byref(st), _ffi.dptr(obs)))

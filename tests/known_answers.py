@@ -89,6 +89,43 @@ def check_channels_bruteforce(eng, kind, n, seed=0):
             assert np.abs(got - want).max() <= TOL
 
 
+def kraus_2q_sets(rng):
+    """2-qubit Kraus sets (local index x_q1 + 2 x_q2): two-qubit depolarizing, amplitude damping
+    on both qubits (kron(K_q2, K_q1)), and a random trace-preserving set."""
+    from linalg_np import mat1
+    P = [np.eye(2), mat1("x"), mat1("y"), mat1("z")]
+    pr = 0.07
+    dep = [np.sqrt(1 - 15 * pr / 16) * np.eye(4)] + [np.sqrt(pr / 16) * np.kron(a, b)
+                                                     for i, a in enumerate(P) for j, b in enumerate(P) if i or j]
+    ad = [np.kron(a, b) for a in kraus_amp_damp(0.05) for b in kraus_amp_damp(0.2)]
+    rnd = rng.standard_normal((3, 4, 4)) + 1j * rng.standard_normal((3, 4, 4))
+    # made trace preserving (K_a <- K_a M^{-1/2}, M = sum_a K_a^dagger K_a) so that repeated
+    # application keeps entries of order one
+    M = sum(K.conj().T @ K for K in rnd)
+    w, V = np.linalg.eigh(M)
+    rnd = [K @ (V @ np.diag(w ** -0.5) @ V.conj().T) for K in rnd]
+    return [dep, ad, rnd]
+
+
+def check_channel2_bruteforce(eng, kind, n, seed=0):
+    """channel_2q against sum_a K_a rho K_a^dagger on dense matrices, every ordered qubit pair."""
+    from linalg_np import embed
+    rng = np.random.default_rng(seed)
+    for ks in kraus_2q_sets(rng):
+        S = superop(ks)
+        for q1 in range(n):
+            for q2 in range(n):
+                if q1 == q2:
+                    continue
+                st, rho = load(eng, kind, n, seed + 7 * q1 + q2)
+                eng.channel2(st, S, q1, q2)
+                F = [embed(np.asarray(K, complex), [q1, q2], n) for K in ks]
+                want = sum(f @ rho @ f.conj().T for f in F)
+                got = logical(eng, st, kind, n)
+                eng.free(st)
+                assert np.abs(got - want).max() <= TOL * max(1.0, np.abs(want).max()), (q1, q2)
+
+
 def gate_full_k(K, q, n):
     from linalg_np import embed
     return embed(np.asarray(K, complex), [q], n)

@@ -101,6 +101,14 @@ CASES = [
      "orkan: channel: channel_1q: superoperator is not 1-local"),
     (1, 2, "so=_ffi.SuperopT(1, None); lib.channel_1q(C.byref(st), C.byref(so), 0)",
      "orkan: channel: channel_1q: null superoperator or data pointer"),
+    (0, 3, "so=_ffi.SuperopT(2, C.addressof(obs)); lib.channel_2q(C.byref(st), C.byref(so), 0, 1)",
+     "orkan: channel: channel_2q: not available for pure states"),
+    (1, 3, "so=_ffi.SuperopT(1, C.addressof(obs)); lib.channel_2q(C.byref(st), C.byref(so), 0, 1)",
+     "orkan: channel: channel_2q: superoperator is not 2-local"),
+    (2, 3, "so=_ffi.SuperopT(2, C.addressof(obs)); lib.channel_2q(C.byref(st), C.byref(so), 1, 1)",
+     "orkan: channel: channel_2q: qubits must differ"),
+    (2, 3, "so=_ffi.SuperopT(2, C.addressof(obs)); lib.channel_2q(C.byref(st), C.byref(so), 0, 5)",
+     "orkan: channel: channel_2q: qubit out of range"),
     (1, 2, "lib.mean(C.byref(st), None)", "orkan: meas: mean: null observable pointer"),
     (1, 2, "lib.matel_diag(C.byref(st), C.byref(st), obs)", "orkan: meas: matel_diag: bra must be PURE (mixed not supported)"),
     (0, 2, "st2=_ffi.StateT(0, fake.value, 3); lib.matel_diag(C.byref(st), C.byref(st2), obs)",

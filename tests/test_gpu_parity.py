@@ -79,6 +79,26 @@ def test_channels(pair, kind):
         dev.free(sd); orc.free(so)
 
 
+@pytest.mark.parametrize("kind", [MIXED_PACKED, MIXED_TILED])
+def test_channel_2q(pair, kind):
+    """channel_2q (no reference counterpart; the oracle's is pinned to sum_a K_a rho K_a^dagger)
+    on every ordered qubit pair."""
+    dev, orc = pair
+    rng = np.random.default_rng(3)
+    sops = [superop(ks) for ks in ka.kraus_2q_sets(rng)]
+    for n in [2, 3] + _sizes(kind, dev.log_tile)[3:]:
+        data = _data(kind, n, 300 + n, dev.log_tile)
+        sd, so = dev.new(kind, n, data), orc.new(kind, n, data)
+        for S in sops:
+            for q1 in range(n):
+                for q2 in range(n):
+                    if q1 != q2:
+                        dev.channel2(sd, S, q1, q2)
+                        orc.channel2(so, S, q1, q2)
+                        _same(dev, orc, sd, so, (n, q1, q2))
+        dev.free(sd); orc.free(so)
+
+
 @pytest.mark.parametrize("kind", [PURE, MIXED_PACKED, MIXED_TILED])
 def test_mean_exp_diag_plus(pair, kind):
     dev, orc = pair

@@ -86,3 +86,37 @@ def test_layout_roundtrip():
             for n in (1, 2, 4, 6):
                 rho = random_rho(n, rng)
                 assert np.allclose(from_layout(to_layout(rho, kind, lt), kind, n, lt), rho)
+
+
+def test_state_file_errors_without_device(tmp_path):
+    """orkan_state_load rejects missing, foreign, truncated and inconsistent files before it
+    touches the device (QS_ERR_FILE = -5, QS_ERR_FORMAT = -6, QS_ERR_NULL = -1)."""
+    import ctypes as C
+    import struct
+    import orkan_amd
+    lib = orkan_amd.load(5)
+    st = orkan_amd.StateT(0, None, 0)
+    assert lib.orkan_state_load(C.byref(st), str(tmp_path / "missing.bin").encode()) == -5
+    assert lib.orkan_state_load(None, b"x") == -1
+    assert lib.orkan_state_save(None, b"x") == -1
+
+    def header(magic=b"ORKAN-ST", version=1, typ=0, qubits=3, lt=5, length=8, eb=16, bom=0x01020304):
+        return magic + struct.pack("<IIIIQII", version, typ, qubits, lt, length, eb, bom) + bytes(24)
+
+    cases = {
+        "short.bin": b"ORKAN-ST" + bytes(10),
+        "magic.bin": header(magic=b"NOTORKAN") + bytes(8 * 16),
+        "version.bin": header(version=2) + bytes(8 * 16),
+        "bom.bin": header(bom=0x04030201) + bytes(8 * 16),
+        "type.bin": header(typ=7) + bytes(8 * 16),
+        "elem.bin": header(eb=8) + bytes(8 * 16),
+        "length.bin": header(length=9) + bytes(9 * 16),
+        "tile.bin": header(typ=2, qubits=5, lt=3, length=state_len(2, 5, 5)) + bytes(state_len(2, 5, 5) * 16),
+        "truncated.bin": header() + bytes(7 * 16),
+        "long.bin": header() + bytes(9 * 16),
+    }
+    for name, blob in cases.items():
+        p = tmp_path / name
+        p.write_bytes(blob)
+        assert lib.orkan_state_load(C.byref(st), str(p).encode()) == -6, name
+        assert st.data is None and st.qubits == 0, name

@@ -30,6 +30,15 @@ def test_channels_bruteforce(log_tile, kind):
 
 
 @pytest.mark.parametrize("log_tile", [3, 5])
+@pytest.mark.parametrize("kind", [MIXED_PACKED, MIXED_TILED])
+def test_channel_2q_bruteforce(log_tile, kind):
+    """channel_2q has no reference counterpart: pinned to sum_a K_a rho K_a^dagger."""
+    eng = engines.oracle(log_tile)
+    for n in (2, 3, log_tile + 1):
+        ka.check_channel2_bruteforce(eng, kind, n, seed=n)
+
+
+@pytest.mark.parametrize("log_tile", [3, 5])
 @pytest.mark.parametrize("kind", ka.KINDS)
 def test_closed_forms(log_tile, kind):
     eng = engines.oracle(log_tile)
