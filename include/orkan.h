@@ -7,4 +7,5 @@
 #include "channel.h"
 #include "meas.h"
 #include "circ.h"
+#include "sample.h"   /* extension: sample_matrix (specified, not implemented, by the reference) */
 #endif /* ORKAN_H */

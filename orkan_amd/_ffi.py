@@ -39,6 +39,13 @@ class SuperopT(C.Structure):
     _fields_ = [("n_qubits", C.c_ubyte), ("data", C.c_void_p)]
 
 
+QOP = C.CFUNCTYPE(None, C.POINTER(StateT))
+
+
+class AnsatzChannelT(C.Structure):
+    _fields_ = [("n_ops", C.c_uint64), ("ops", C.POINTER(QOP)), ("weights", C.c_void_p)]
+
+
 PS = C.POINTER(StateT)
 Q = C.c_ubyte
 D = C.c_double
@@ -66,6 +73,8 @@ API = {
     "kraus_to_superop": (SuperopT, [C.POINTER(KrausT)]),
     "channel_1q": (None, [PS, C.POINTER(SuperopT), Q]),
     "channel_2q": (None, [PS, C.POINTER(SuperopT), Q, Q]),
+    "sample_matrix": (C.c_int, [PS, C.c_void_p, C.POINTER(AnsatzChannelT), C.c_uint64, C.c_uint64, C.c_size_t,
+                                C.c_void_p]),
     "mean": (D, [PS, DP]),
     "matel_diag": (Cplx, [PS, PS, DP]),
     "exp_diag": (None, [PS, DP, D]),

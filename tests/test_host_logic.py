@@ -120,3 +120,32 @@ def test_state_file_errors_without_device(tmp_path):
         p.write_bytes(blob)
         assert lib.orkan_state_load(C.byref(st), str(p).encode()) == -6, name
         assert st.data is None and st.qubits == 0, name
+
+
+def test_sample_matrix_validation_without_device():
+    """sample_matrix rejects bad arguments before it touches the device (QS_ERR_NULL -1,
+    QS_ERR_TYPE -4, QS_ERR_PARAM -7)."""
+    import ctypes as C
+    import orkan_amd
+    from orkan_amd import _ffi
+    lib = orkan_amd.load(5)
+    fake = C.c_void_p(0x1000)
+    st = _ffi.StateT(0, fake.value, 3)
+    obs = (C.c_double * 8)()
+    noop = _ffi.QOP(lambda s: None)
+    ops = (_ffi.QOP * 2)(noop, noop)
+    w = (C.c_double * 4)(1, 0, 1, 0)
+    ch = (_ffi.AnsatzChannelT * 2)(_ffi.AnsatzChannelT(2, ops, C.addressof(w)), _ffi.AnsatzChannelT(2, ops, None))
+    S = (C.c_double * 8)()
+    big = 1 << 20
+    call = lambda s, o, c, L, ls, b, out: lib.sample_matrix(s, o, c, L, ls, b, out)
+    assert call(None, obs, ch, 2, 1, big, S) == -1
+    assert call(C.byref(st), None, ch, 2, 1, big, S) == -1
+    assert call(C.byref(st), obs, ch, 2, 1, big, None) == -1
+    st_mixed = _ffi.StateT(1, fake.value, 3)
+    assert call(C.byref(st_mixed), obs, ch, 2, 1, big, S) == -4
+    assert call(C.byref(st), obs, ch, 2, 2, big, S) == -7            # lstar out of range
+    assert call(C.byref(st), obs, ch, 2, 0, big, S) == -7            # channel 1 has no weights
+    assert call(C.byref(st), obs, ch, 2, 1, 3 * 16 * 8, S) == -7     # workspace below 4 vectors
+    empty = (_ffi.AnsatzChannelT * 1)(_ffi.AnsatzChannelT(0, ops, None))
+    assert call(C.byref(st), obs, empty, 1, 0, big, S) == -7
