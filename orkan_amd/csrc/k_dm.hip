@@ -307,24 +307,40 @@ __device__ __forceinline__ void sop4(const DmOp &op, cd v0, cd v1, cd v2, cd v3,
 
 __device__ __forceinline__ u64 tile_base(u64 TR, u64 TC, int L) { return (TR * (TR + 1) / 2 + TC) << (2 * L); }
 
-template <typename S>
+template <typename S, int U>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     const int L = sp.L, q = sp.m, hl = L - 1;
     const int mb = 1 << q;
     const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
-    const u64 Q = w * DM_THREADS + threadIdx.x;
-    if (Q >= sp.nitems) return;
-    const int i = (int)(Q & (((u64)1 << (2 * hl)) - 1));
-    const int r0 = (int)ins0((u64)(i >> hl), q), c0 = (int)ins0((u64)(i & ((1 << hl) - 1)), q);
-    S *t = rho + ((Q >> (2 * hl)) << (2 * L));
-    const int a00 = (r0 << L) | c0, a01 = a00 | mb, a10 = a00 | (mb << L), a11 = a10 | mb;
-    const cd v0 = ldc(t + a00), v1 = ldc(t + a01), v2 = ldc(t + a10), v3 = ldc(t + a11);
-    cd o0, o1, o2, o3;
-    sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
-    stc(t + a00, o0);
-    stc(t + a01, o1);
-    stc(t + a10, o2);
-    stc(t + a11, o3);
+    S *t[U];
+    int a00[U];
+    cd v[U][4];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 Q = (w * U + u) * DM_THREADS + threadIdx.x;
+        ok[u] = Q < sp.nitems;
+        const int i = (int)(Q & (((u64)1 << (2 * hl)) - 1));
+        const int r0 = (int)ins0((u64)(i >> hl), q), c0 = (int)ins0((u64)(i & ((1 << hl) - 1)), q);
+        t[u] = ok[u] ? rho + ((Q >> (2 * hl)) << (2 * L)) : rho;
+        a00[u] = (r0 << L) | c0;
+        if (ok[u]) {
+            v[u][0] = ldc(t[u] + a00[u]);
+            v[u][1] = ldc(t[u] + (a00[u] | mb));
+            v[u][2] = ldc(t[u] + (a00[u] | (mb << L)));
+            v[u][3] = ldc(t[u] + (a00[u] | (mb << L) | mb));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        cd o0, o1, o2, o3;
+        sop4(op, v[u][0], v[u][1], v[u][2], v[u][3], o0, o1, o2, o3);
+        stc(t[u] + a00[u], o0);
+        stc(t[u] + (a00[u] | mb), o1);
+        stc(t[u] + (a00[u] | (mb << L)), o2);
+        stc(t[u] + (a00[u] | (mb << L) | mb), o3);
+    }
 }
 
 template <typename S>
@@ -636,11 +652,13 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     sp.xcd = sh.xcd;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
     if (q < L) {
+        // complex64 storage: two quads per thread and half the LDS cap keep the bytes in flight
+        constexpr int U = sizeof(S) == 8 ? 2 : 1;
         sp.m = q;
         sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 2);
-        sp.nblocks = (sp.nitems + DM_THREADS - 1) / DM_THREADS;
+        sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        dm_sop_intile_kernel<S><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds, st>>>((S *)rho, sp, op);
+        dm_sop_intile_kernel<S, U><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
     } else {
         sp.m = q - L;
         const u64 h = nt >> 1;

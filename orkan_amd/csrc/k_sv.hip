@@ -11,7 +11,7 @@
 namespace {
 
 constexpr int SV_THREADS = 256;
-constexpr int SV_UNROLL = 2;      // bases per thread in the permutation kernel
+constexpr int SV_UNROLL = 2;      // bases per thread in the permutation kernel (doubled for complex64)
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 2048;  // first-stage partial sums (fixed: deterministic order)
 
@@ -83,7 +83,7 @@ __device__ __forceinline__ cd one_op(cd b, const Coef &k) {
 // is capped by requesting unused dynamic LDS.
 struct SvShape {
     int xcd;   // remap block ids so consecutive chunks run on the same XCD
-    int u;     // pairs per thread (1 or 2)
+    int u;     // pairs per thread (1 or 2; doubled for complex64 storage)
     int lds;   // dynamic LDS bytes per workgroup (0: no cap)
 };
 
@@ -165,14 +165,14 @@ struct MonoArgs {
     int ph[8];        // times i^ph
 };
 
-template <typename S>
+template <typename S, int U>
 __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi, u64 nbase, MonoArgs m) {
-    const u64 b0 = chunk_of(1) * (SV_THREADS * SV_UNROLL) + threadIdx.x;
+    const u64 b0 = chunk_of(1) * (SV_THREADS * U) + threadIdx.x;
     // fixed-bound loops with guards keep v[][] in registers (no dynamic indexing)
-    cd v[SV_UNROLL][8];
-    u64 base[SV_UNROLL];
+    cd v[U][8];
+    u64 base[U];
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 b = b0 + (u64)u * SV_THREADS;
         base[u] = ins0k(b, m.pos, m.k);
         if (b < nbase) {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
         }
     }
 #pragma unroll
-    for (int u = 0; u < SV_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) {
         const u64 b = b0 + (u64)u * SV_THREADS;
         if (b < nbase) {
 #pragma unroll
@@ -232,14 +232,29 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
+// Four independent accumulators per thread: x, x + G, x + 2G, x + 3G (G = grid stride) are loaded
+// together, so each thread keeps four loads in flight; they are added in that fixed order.
+constexpr int RED_UNROLL = 4;
+
 template <typename S>
 __global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restrict__ psi, const double *__restrict__ obs,
                                                               u64 dim, double *__restrict__ part) {
-    double acc[1] = {0.0};
-    for (u64 x = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x < dim; x += (u64)gridDim.x * RED_THREADS) {
-        const cd a = ldc(psi + x);
-        acc[0] += obs[x] * (a.x * a.x + a.y * a.y);
+    double a4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0};
+    const u64 G = (u64)gridDim.x * RED_THREADS;
+    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * RED_UNROLL) {
+        cd a[RED_UNROLL];
+        double o[RED_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RED_UNROLL; ++u) {
+            const u64 x = x0 + (u64)u * G;
+            const bool in = x < dim;
+            a[u] = in ? ldc(psi + x) : cmk(0.0, 0.0);
+            o[u] = in ? obs[x] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_UNROLL; ++u) a4[u] += o[u] * (a[u].x * a[u].x + a[u].y * a[u].y);
     }
+    double acc[1] = {((a4[0] + a4[1]) + a4[2]) + a4[3]};
     block_sum<1>(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
@@ -248,13 +263,26 @@ template <typename S>
 __global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restrict__ bra, const S *__restrict__ ket,
                                                                const double *__restrict__ obs, u64 dim,
                                                                double *__restrict__ part) {
-    double acc[2] = {0.0, 0.0};
-    for (u64 x = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x < dim; x += (u64)gridDim.x * RED_THREADS) {
-        const cd a = ldc(bra + x), b = ldc(ket + x);
-        const double o = obs[x];
-        acc[0] += o * (a.x * b.x + a.y * b.y);
-        acc[1] += o * (a.x * b.y - a.y * b.x);
+    double re4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0}, im4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0};
+    const u64 G = (u64)gridDim.x * RED_THREADS;
+    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * RED_UNROLL) {
+        cd a[RED_UNROLL], b[RED_UNROLL];
+        double o[RED_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RED_UNROLL; ++u) {
+            const u64 x = x0 + (u64)u * G;
+            const bool in = x < dim;
+            a[u] = in ? ldc(bra + x) : cmk(0.0, 0.0);
+            b[u] = in ? ldc(ket + x) : cmk(0.0, 0.0);
+            o[u] = in ? obs[x] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_UNROLL; ++u) {
+            re4[u] += o[u] * (a[u].x * b[u].x + a[u].y * b[u].y);
+            im4[u] += o[u] * (a[u].x * b[u].y - a[u].y * b[u].x);
+        }
     }
+    double acc[2] = {((re4[0] + re4[1]) + re4[2]) + re4[3], ((im4[0] + im4[1]) + im4[2]) + im4[3]};
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = acc[0];
@@ -291,9 +319,18 @@ void launch_shape(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, i
 template <typename S, int OP>
 void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
     SvShape sh = pair_shape(q);
+    // complex64 storage moves half the bytes per pair: twice the pairs per thread and twice the
+    // resident workgroups keep the same bytes in flight per CU as the complex128 shape
+    if (sizeof(S) == 8) {
+        sh.u *= 2;
+        sh.lds /= 2;
+    }
     // the remap needs a grid of at least 8 blocks; small states use the plain order
     if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
-    if (sh.u == 2) {
+    if (sh.u == 4) {
+        if (sh.xcd) launch_shape<S, OP, 4, 1>(st, psi, npairs, q, k, sh.lds);
+        else launch_shape<S, OP, 4, 0>(st, psi, npairs, q, k, sh.lds);
+    } else if (sh.u == 2) {
         if (sh.xcd) launch_shape<S, OP, 2, 1>(st, psi, npairs, q, k, sh.lds);
         else launch_shape<S, OP, 2, 0>(st, psi, npairs, q, k, sh.lds);
     } else {
@@ -362,12 +399,17 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     }
     if (m.nt == 0) return 0;
     const u64 nbase = ((u64)1 << n) >> m.k;
-    const u64 per = (u64)SV_THREADS * SV_UNROLL;
-    const unsigned grid = (unsigned)((nbase + per - 1) / per);
     hipStream_t st = (hipStream_t)stream;
     const int lds = 53248;   // same residency cap as the pair kernel
-    if (prec == ORK_F32) sv_mono_kernel<float2><<<grid, SV_THREADS, lds, st>>>((float2 *)psi, nbase, m);
-    else sv_mono_kernel<double2><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
+    if (prec == ORK_F32) {
+        const u64 per = (u64)SV_THREADS * SV_UNROLL * 2;
+        const unsigned grid = (unsigned)((nbase + per - 1) / per);
+        sv_mono_kernel<float2, SV_UNROLL * 2><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
+    } else {
+        const u64 per = (u64)SV_THREADS * SV_UNROLL;
+        const unsigned grid = (unsigned)((nbase + per - 1) / per);
+        sv_mono_kernel<double2, SV_UNROLL><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
+    }
     ORK_LAUNCH_CHECK();
     return 0;
 }
