@@ -79,7 +79,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
 /* tuning hook: launch shape of the density-matrix kernels (kernel 0: tile groups of both layouts,
  * 1: phases of both layouts, 2 / 3: tile groups of the packed / tiled layout, 4 / 5: phases of the
  * packed / tiled layout, 6 / 7: streamed tiled 1-qubit superoperator with the target inside / above
- * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups);
+ * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups,
+ * 9: streamed packed superoperator, 10: statevector mean / matel_diag reduction (lds_bytes only));
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column;
  * lds_bytes caps workgroups per CU (0: no cap) */
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);

@@ -181,3 +181,33 @@ def spearman(a, b):
     ra = np.argsort(np.argsort(a)).astype(np.float64)
     rb = np.argsort(np.argsort(b)).astype(np.float64)
     return float(np.corrcoef(ra, rb)[0, 1])
+
+
+def screen_only(lib, ncand=1_000_000, reps=6, topk=64):
+    """The screening path alone (random weights): forward over one pool of candidates and the
+    device top-k, `reps` times; for kernel traces of the screening forward."""
+    rng = wl.rng(1)
+    mlp = surrogate.Mlp(lib, max_batch=65536)
+    mlp.init(rng)
+    pool = Pool(lib, rng.uniform(0, np.pi, (ncand, P)), rng.uniform(0, np.pi, (ncand, P)))
+    gf = surrogate.graph_features(rng.uniform(0, 1, 60))
+    for r in range(reps):
+        t0 = time.perf_counter()
+        pool.rank(mlp, gf, topk)
+        log(f"screen: {ncand} candidates ranked in {(time.perf_counter() - t0) * 1e3:.2f} ms")
+    pool.close()
+    mlp.close()
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="learned screening benchmark")
+    ap.add_argument("--screen", action="store_true", help="time the screening forward alone")
+    a = ap.parse_args()
+    import orkan_amd
+    lib = orkan_amd.load(5)
+    if a.screen:
+        screen_only(lib)
+    else:
+        import json
+        print(json.dumps(run(lib)))

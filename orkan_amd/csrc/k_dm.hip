@@ -357,8 +357,25 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     S *C = rho + tile_base(TR | bb, TC, L);
     S *D = rho + tile_base(TR | bb, TC | bb, L);
     if ((TR >> (sp.m + 1)) != (TC >> (sp.m + 1))) {
-        // all four tiles stored as themselves: two in-tile entries per thread in flight
+        // all four tiles stored as themselves: with 32 x 32 tiles every thread loads its four
+        // in-tile entries of all four tiles at once (64 KB of a workgroup in flight)
         S *B = rho + tile_base(TR, TC | bb, L);
+        if (TT == 4 * DM_THREADS) {
+            cd va[4], vb[4], vc[4], vd[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int o = tid + e * DM_THREADS;
+                va[e] = ldc(A + o); vb[e] = ldc(B + o); vc[e] = ldc(C + o); vd[e] = ldc(D + o);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int o = tid + e * DM_THREADS;
+                cd p0, p1, p2, p3;
+                sop4(op, va[e], vb[e], vc[e], vd[e], p0, p1, p2, p3);
+                stc(A + o, p0); stc(B + o, p1); stc(C + o, p2); stc(D + o, p3);
+            }
+            return;
+        }
         for (int o = tid; o < TT; o += 2 * DM_THREADS) {
             const int o2 = o + DM_THREADS;
             const bool two = o2 < TT;
@@ -430,14 +447,15 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
         }
         __syncthreads();
     }
-    // two quads per thread in flight: t and t + 512
-    for (int t = tid; t < 512; t += DM_THREADS) {
-        cd v[2][4];
-        S *p0[2], *p1[2], *p2[2], *p3[2];
-        int kind[2];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
+    // all four quads of a thread in flight: t = tid + 256 h
+    {
+        constexpr int QPT = 1024 / DM_THREADS;
+        cd v[QPT][4];
+        S *p0[QPT], *p1[QPT], *p2[QPT], *p3[QPT];
+        int kind[QPT];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int tt = t + 512 * h, x = tt & 31, y = tt >> 5;   // lanes walk x: rows r0 of column c0(y)
+        for (int h = 0; h < QPT; ++h) {
+            const int tt = tid + DM_THREADS * h, x = tt & 31, y = tt >> 5;   // lanes walk x: rows r0 of column c0(y)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             kind[h] = a < b ? 0 : a == b ? 3 : ((a >> q) == (b >> q) ? 2 : 1);
             p0[h] = p1[h] = p2[h] = p3[h] = rho;
@@ -456,9 +474,9 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < QPT; ++h) {
             if (!kind[h]) continue;
-            const int tt = t + 512 * h, x = tt & 31, y = tt >> 5;
+            const int tt = tid + DM_THREADS * h, x = tt & 31, y = tt >> 5;
             cd o0, o1, o2, o3;
             sop4(op, v[h][0], v[h][1], v[h][2], v[h][3], o0, o1, o2, o3);
             stc(p0[h], o0);

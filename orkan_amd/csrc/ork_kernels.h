@@ -83,6 +83,8 @@ int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rh
 /* launch shape of the density-matrix kernels: which 0 = tile-group kernel, 1 = phase kernel;
  * xcd: keep consecutive groups on one XCD; lds_bytes: LDS per workgroup (caps residency) */
 int ork_dm_set_shape(int which, int flags, int lds_bytes);
+/* residency cap (dynamic LDS bytes per workgroup) of the statevector mean / matel reduction */
+int ork_sv_set_red_lds(int lds_bytes);
 
 /* number of doubles the *_mean / matel launchers need in partials_dev */
 uint64_t ork_partials_len(void);
