@@ -138,6 +138,21 @@ def run_sv28(args, lib):
     mval = lib.mean(ps, C.cast(obs_dev, C.POINTER(C.c_double)))
     m1 = ev.rec()
     mean_ms = ev.ms(m0, m1)
+    # after the circuit: matel_diag against a copy (reads both states and obs, 40 B per amplitude) and
+    # exp_diag with obs as the diagonal (reads psi and the diagonal, writes psi: 40 B per amplitude)
+    obs_p = C.cast(obs_dev, C.POINTER(C.c_double))
+    other = lib.state_cp(ps)
+    lib.orkan_sync()
+    x0 = ev.rec()
+    lib.matel_diag(ps, C.byref(other), obs_p)
+    x1 = ev.rec()
+    lib.state_free(C.byref(other))
+    lib.exp_diag(ps, obs_p, 0.3)
+    e0 = ev.rec()
+    lib.exp_diag(ps, obs_p, 0.3)
+    e1 = ev.rec()
+    lib.orkan_sync()
+    matel_ms, expd_ms = ev.ms(x0, x1), ev.ms(e0, e1)
 
     if args.copy_calibration:
         cp = lib.orkan_state_convert(ps, 1 if args.fp32 else 0)
@@ -153,7 +168,9 @@ def run_sv28(args, lib):
         per_gate[name] = round(rot_bytes / (float(rot_ms[rot_names == name].mean()) * 1e-3) / 1e9, 1)
     cx_bytes = wl.sv_gate_bytes("cx", n, elem)
     per_gate["cx"] = round(cx_bytes / (float(cx_ms.mean()) * 1e-3) / 1e9, 1)
-    per_gate["mean"] = round(24 * (1 << n) * (elem / 16) / (mean_ms * 1e-3) / 1e9, 1) if mean_ms > 0 else None
+    per_gate["mean"] = round((elem + 8) * (1 << n) / (mean_ms * 1e-3) / 1e9, 1) if mean_ms > 0 else None
+    per_gate["matel_diag"] = round((2 * elem + 8) * (1 << n) / (matel_ms * 1e-3) / 1e9, 1) if matel_ms > 0 else None
+    per_gate["exp_diag"] = round((2 * elem + 8) * (1 << n) / (expd_ms * 1e-3) / 1e9, 1) if expd_ms > 0 else None
 
     value = total_bytes / (total_ms * 1e-3) / 1e9
     lib.orkan_dev_free(obs_dev)

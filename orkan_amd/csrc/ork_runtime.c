@@ -13,7 +13,9 @@
 
 static ork_dev_t g_dev[ORK_MAX_DEV];
 static int g_ready[ORK_MAX_DEV];
-static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;        /* ork_lock: shared staging, partials */
+static pthread_mutex_t g_init_mu = PTHREAD_MUTEX_INITIALIZER;   /* per-device runtime creation, so that
+                                                                 * ork_dev() may be called under ork_lock */
 
 void ork_hip_fail(const char *where, int err) {
     fprintf(stderr, "orkan: hip: %s: %s\n", where, hipGetErrorString((hipError_t)err));
@@ -27,18 +29,18 @@ ork_dev_t *ork_dev(void) {
     int dev = 0;
     ORK_HIP("hipGetDevice", hipGetDevice(&dev));
     if (dev < 0 || dev >= ORK_MAX_DEV) ork_hip_fail("device index", (int)hipErrorInvalidDevice);
-    if (!g_ready[dev]) {
-        pthread_mutex_lock(&g_mu);
-        if (!g_ready[dev]) {
+    if (!__atomic_load_n(&g_ready[dev], __ATOMIC_ACQUIRE)) {
+        pthread_mutex_lock(&g_init_mu);
+        if (!__atomic_load_n(&g_ready[dev], __ATOMIC_ACQUIRE)) {
             ork_dev_t *d = &g_dev[dev];
             memset(d, 0, sizeof(*d));
             hipStream_t st;
             ORK_HIP("hipStreamCreateWithFlags", hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             d->stream = (void *)st;
             ORK_HIP("hipMalloc(partials)", hipMalloc((void **)&d->partials, ork_partials_len() * sizeof(double)));
-            g_ready[dev] = 1;
+            __atomic_store_n(&g_ready[dev], 1, __ATOMIC_RELEASE);
         }
-        pthread_mutex_unlock(&g_mu);
+        pthread_mutex_unlock(&g_init_mu);
     }
     return &g_dev[dev];
 }
