@@ -80,7 +80,9 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * 1: phases of both layouts, 2 / 3: tile groups of the packed / tiled layout, 4 / 5: phases of the
  * packed / tiled layout, 6 / 7: streamed tiled 1-qubit superoperator with the target inside / above
  * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups,
- * 9: streamed packed superoperator, 10: statevector mean / matel_diag reduction (lds_bytes only));
+ * 9: streamed packed superoperator, 10: statevector mean / matel_diag reduction (lds_bytes only),
+ * 11: surrogate GEMM tile where the shape allows (flags only: 0 128 x 128, 1 256 x 256 staged
+ * through registers, 2 256 x 256 streamed into LDS by DMA, the default));
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column;
  * lds_bytes caps workgroups per CU (0: no cap) */
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);

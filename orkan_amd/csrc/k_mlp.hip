@@ -234,6 +234,126 @@ __global__ __launch_bounds__(GT2) void gemm_nt_big_kernel(const bf16 *__restrict
 }
 constexpr int BIG_LDS = 2 * (BM2 + BN2) * KP2 * (int)sizeof(bf16);
 
+// ---- 256 x 256 tile, operands streamed into LDS by DMA (global_load_lds), four stages ----------
+// Same waves and MFMA tiles as gemm_nt_big_kernel, but K steps of 32 move from global memory
+// straight into LDS (no staging registers), three steps ahead of the one being multiplied.  Each
+// stage is its own __shared__ array, so the compiler's wait tracking keeps the DMA into one stage
+// from blocking reads of another.  A stage row is 64 B (32 bf16); its four 16 B chunks are stored
+// XOR-swizzled (position p holds K chunk p ^ ((row >> 2) & 3)), which keeps the 16 rows an MFMA
+// operand read touches in distinct banks without padding (a DMA instruction writes 64 contiguous
+// 16 B slots, one per lane).
+constexpr int BK3 = 32, GT3 = 512;
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+
+template <int EPI>
+__global__ __launch_bounds__(GT3) void gemm_nt_dma_kernel(const bf16 *__restrict__ A, int lda,
+                                                          const bf16 *__restrict__ B, int ldb, int K, EpiArgs ep) {
+    __shared__ __attribute__((aligned(16))) bf16 sa0[BM2 * BK3], sa1[BM2 * BK3], sa2[BM2 * BK3], sa3[BM2 * BK3];
+    __shared__ __attribute__((aligned(16))) bf16 sb0[BN2 * BK3], sb1[BN2 * BK3], sb2[BN2 * BK3], sb3[BN2 * BK3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    const long m0 = (long)blockIdx.y * BM2, n0 = (long)blockIdx.x * BN2;
+    // DMA: wave w fills rows [32 w, 32 w + 32) of the A and B stage, 16 rows (1 KB) per instruction;
+    // lane l lands in row 32 w + 16 i + l / 4 at chunk position l % 4
+    const int drow = wave * 32 + (lane >> 2), dp = lane & 3;
+    const bf16 *ga0 = A + (m0 + drow) * (long)lda + swz(drow, dp) * 8;
+    const bf16 *ga1 = A + (m0 + drow + 16) * (long)lda + swz(drow + 16, dp) * 8;
+    const bf16 *gb0 = B + (n0 + drow) * (long)ldb + swz(drow, dp) * 8;
+    const bf16 *gb1 = B + (n0 + drow + 16) * (long)ldb + swz(drow + 16, dp) * 8;
+    const int dst0 = wave * 32 * BK3, dst1 = (wave * 32 + 16) * BK3;
+#define DMA(SA, SB, k0)                                                                         \
+    do {                                                                                        \
+        __builtin_amdgcn_global_load_lds((const void *)(ga0 + (k0)), (lds_void *)((SA) + dst0), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((const void *)(ga1 + (k0)), (lds_void *)((SA) + dst1), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((const void *)(gb0 + (k0)), (lds_void *)((SB) + dst0), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((const void *)(gb1 + (k0)), (lds_void *)((SB) + dst1), 16, 0, 0); \
+    } while (0)
+    f32x16 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int KT = K / BK3;
+    const int l32 = lane & 31, lhi = lane >> 5;
+    int ra[4], rb[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ra[i] = wm * 128 + i * 32 + l32;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) rb[j] = wn * 64 + j * 32 + l32;
+#define MMA(SA, SB)                                                                                    \
+    do {                                                                                               \
+        _Pragma("unroll") for (int s = 0; s < BK3 / 16; ++s) {                                         \
+            const int c = 2 * s + lhi;                                                                 \
+            bf16x8 av[4], bv[2];                                                                       \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                              \
+                av[i] = *(const bf16x8 *)((SA) + ra[i] * BK3 + swz(ra[i], c) * 8);                     \
+            _Pragma("unroll") for (int j = 0; j < 2; ++j)                                              \
+                bv[j] = *(const bf16x8 *)((SB) + rb[j] * BK3 + swz(rb[j], c) * 8);                     \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                              \
+                _Pragma("unroll") for (int j = 0; j < 2; ++j)                                          \
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0); \
+        }                                                                                              \
+    } while (0)
+    // stage kc sits in (CA, CB).  Every step issues one stage of DMA (past the end of K it reloads
+    // the last stage into the slot nobody reads again), so at step kc the stages kc + 1 and kc + 2
+    // are the only later ones in flight: wait until this wave has at most those 8 instructions
+    // outstanding, barrier (every wave's DMA of stage kc landed and every wave is done with stage
+    // kc - 1: its LDS reads fed MFMAs issued before the barrier, and the scheduling barrier keeps
+    // them there), refill the slot of stage kc - 1 with stage kc + 3, multiply stage kc.
+    // s_waitcnt immediate: vmcnt in bits 3:0, expcnt 6:4 = 7 and lgkmcnt 11:8 = 15 (no wait).  The
+    // acquire fence (LDS only) keeps the stage reads below the barrier without the vmcnt(0) that a
+    // release fence adds for DMA still in flight.
+#define STEP(CA, CB, NA, NB, kc)                                                                       \
+    do {                                                                                               \
+        __builtin_amdgcn_s_waitcnt(0x0F70 | 8);                                                        \
+        __builtin_amdgcn_s_barrier();                                                                  \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");                                \
+        DMA(NA, NB, ((kc) + 3 < KT ? (kc) + 3 : KT - 1) * BK3);                                        \
+        MMA(CA, CB);                                                                                   \
+        __builtin_amdgcn_sched_barrier(0);                                                             \
+    } while (0)
+    // K is a multiple of 4 BK3 (the host checks), so the loop body runs whole
+    DMA(sa0, sb0, 0);
+    DMA(sa1, sb1, BK3);
+    DMA(sa2, sb2, 2 * BK3);
+    for (int kt = 0; kt < KT; kt += 4) {
+        STEP(sa0, sb0, sa3, sb3, kt);
+        STEP(sa1, sb1, sa0, sb0, kt + 1);
+        STEP(sa2, sb2, sa1, sb1, kt + 2);
+        STEP(sa3, sb3, sa2, sb2, kt + 3);
+    }
+    // no DMA may still be writing this workgroup's LDS when it ends
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+#undef STEP
+#undef MMA
+#undef DMA
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long row = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
+                const long col = n0 + wn * 64 + j * 32 + l32;
+                const long at = row * ep.ldc + col;
+                const float v = acc[i][j][r];
+                if constexpr (EPI == EPI_F32) ep.c32[at] = v;
+                else if constexpr (EPI == EPI_F32_ACC) ep.c32[at] += v;
+                else if constexpr (EPI == EPI_BIAS_GELU) {
+                    const float z = v + ep.bias[col];
+                    if (ep.pre) ep.pre[at] = z;
+                    ep.act[at] = (bf16)gelu(z);
+                } else {
+                    ep.act[at] = (bf16)(v * gelu_grad(ep.pre[at]));
+                }
+            }
+}
+
+
 // ---- small kernels -----------------------------------------------------------------------------
 __global__ void transpose_bf16_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int R, int Cc) {
     __shared__ bf16 t[32][33];
@@ -477,8 +597,12 @@ unsigned g1(long n) {
 }  // namespace
 
 // ---- C ABI ------------------------------------------------------------------------------------
-static int g_gemm_big = 1;   // use the 256 x 256 tile where the shape allows
-extern "C" int ork_mlp_gemm_select(int big) { g_gemm_big = big ? 1 : 0; return 0; }
+static int g_gemm_big = 2;   // 256 x 256 tiles where the shape allows: 1 register-staged, 2 DMA-staged
+extern "C" int ork_mlp_gemm_select(int variant) {
+    if (variant < 0 || variant > 2) return (int)hipErrorInvalidValue;
+    g_gemm_big = variant;
+    return 0;
+}
 
 // epi: 0 fp32 C, 1 bias+GELU, 2 GELU backward, 3 fp32 C +=
 extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
@@ -487,6 +611,18 @@ extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const
     EpiArgs ep{c32, ldc, bias, pre, (bf16 *)act};
     hipStream_t st = (hipStream_t)stream;
     const bf16 *a = (const bf16 *)A, *b = (const bf16 *)B;
+    if (g_gemm_big == 2 && M % BM2 == 0 && N % BN2 == 0 && K % (4 * BK3) == 0 && K > 0) {
+        dim3 grid3(N / BN2, M / BM2);
+        switch (epi) {
+        case EPI_F32: gemm_nt_dma_kernel<EPI_F32><<<grid3, GT3, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_BIAS_GELU: gemm_nt_dma_kernel<EPI_BIAS_GELU><<<grid3, GT3, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_GELU_BWD: gemm_nt_dma_kernel<EPI_GELU_BWD><<<grid3, GT3, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        case EPI_F32_ACC: gemm_nt_dma_kernel<EPI_F32_ACC><<<grid3, GT3, 0, st>>>(a, lda, b, ldb, K, ep); break;
+        default: return (int)hipErrorInvalidValue;
+        }
+        ORK_LAUNCH_CHECK();
+        return 0;
+    }
     if (g_gemm_big && M % BM2 == 0 && N % BN2 == 0 && K % BK2 == 0) {
         dim3 grid2(N / BN2, M / BM2);
         switch (epi) {

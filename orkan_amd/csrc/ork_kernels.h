@@ -85,6 +85,7 @@ int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rh
 int ork_dm_set_shape(int which, int flags, int lds_bytes);
 /* residency cap (dynamic LDS bytes per workgroup) of the statevector mean / matel reduction */
 int ork_sv_set_red_lds(int lds_bytes);
+int ork_mlp_gemm_select(int variant);
 
 /* number of doubles the *_mean / matel launchers need in partials_dev */
 uint64_t ork_partials_len(void);

@@ -99,6 +99,7 @@ int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count) {
 
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes) {
     if (kernel == 10) return ork_sv_set_red_lds(lds_bytes);
+    if (kernel == 11) return ork_mlp_gemm_select(flags);
     return ork_dm_set_shape(kernel, flags, lds_bytes);
 }
 

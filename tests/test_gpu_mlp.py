@@ -28,10 +28,17 @@ def upload(lib, arr):
     return p
 
 
+GEMM_TILE_HOOK = 11
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 96), (512, 128, 512), (256, 256, 64),
-                                   (512, 512, 512)])
-def test_gemm_exact_small_integers(M, N, K):
+                                   (512, 512, 512), (256, 512, 128), (768, 256, 1024)])
+def test_gemm_exact_small_integers(M, N, K, variant):
+    """Every tile variant the shape admits (128 x 128; 256 x 256 register-staged when M, N % 256
+    and K % 64; 256 x 256 DMA-staged when M, N % 256 and K % 128; otherwise the next smaller)."""
     lib = orkan_amd.load(5)
+    assert lib.orkan_set_launch_shape(GEMM_TILE_HOOK, variant, 0) == 0
     rng = np.random.default_rng(M + N + K)
     A = rng.integers(-3, 4, (M, K)).astype(np.float32)
     B = rng.integers(-3, 4, (N, K)).astype(np.float32)
@@ -40,6 +47,7 @@ def test_gemm_exact_small_integers(M, N, K):
     assert lib.orkan_mlp_gemm_test(M, N, K, pa, pb, pc) == 0
     got = np.empty(M * N, np.float32)
     assert lib.orkan_copy_to_host(got.ctypes.data, pc, got.nbytes) == 0
+    assert lib.orkan_set_launch_shape(GEMM_TILE_HOOK, 2, 0) == 0
     assert np.array_equal(got.reshape(M, N), (A @ B.T).astype(np.float32))
     for p in (pa, pb, pc):
         lib.orkan_dev_free(p)
