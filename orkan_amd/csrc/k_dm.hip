@@ -394,6 +394,46 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     }
     const bool diag = (TR == TC);
     S *X = diag ? C : rho + tile_base(TC | bb, TR, L);
+    if (TT == 4 * DM_THREADS) {
+        // 32 x 32 tiles: every load of the workgroup (A, D, X and, off the diagonal, C) in flight at
+        // once.  On the diagonal X is C itself, so a thread's own C entry comes from its registers
+        // and only the transposed one from LDS.
+        cd va[4], vc[4], vd[4], vx[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int o = tid + e * DM_THREADS;
+            va[e] = ldc(A + o); vd[e] = ldc(D + o); vx[e] = ldc(X + o);
+            if (!diag) vc[e] = ldc(C + o);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int o = tid + e * DM_THREADS;
+            xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = vx[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int o = tid + e * DM_THREADS, i = o >> L, j = o & (T - 1);
+            // (j, i) of the LDS copy is read and then rewritten by this thread alone
+            const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
+            const cd v2 = diag ? vx[e] : vc[e];
+            cd o0, o1, o2, o3;
+            sop4(op, va[e], v1, v2, vd[e], o0, o1, o2, o3);
+            stc(A + o, o0);
+            stc(C + o, o2);
+            stc(D + o, o3);
+            if (!diag) xs[j * LDS_STRIDE + i] = cconj(o1);
+        }
+        if (!diag) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int o = tid + e * DM_THREADS;
+                stc(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+            }
+        }
+        return;
+    }
     for (int o = tid; o < TT; o += DM_THREADS) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldc(X + o);
     __syncthreads();
     for (int o = tid; o < TT; o += DM_THREADS) {
