@@ -478,18 +478,21 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
     u64 A, Bk;
     block_rc(blockIdx.x, pp.nblocks, pp.nt, pp.xcd, pp.col, A, Bk);
     const bool has_mirror = q >= 5 ? ((A >> (q - 5)) == (Bk >> (q - 5))) : (A == Bk);
+    constexpr int QPT = 1024 / DM_THREADS;
+    // the block's mirror entries, loaded together with the quads below and staged through LDS
+    // once every load is in flight
+    cd mv[QPT];
     if (has_mirror) {
-        for (int t = tid; t < 1024; t += DM_THREADS) {
-            const int x = t >> 5, y = t & 31;          // lanes walk y: rows c1 of column r0(x)
+#pragma unroll
+        for (int h = 0; h < QPT; ++h) {
+            const int t = tid + DM_THREADS * h, x = t >> 5, y = t & 31;   // lanes walk y: rows c1 of column r0(x)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
-                ms[x * LDS_STRIDE + y] = ldc(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
+                mv[h] = ldc(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
-        __syncthreads();
     }
     // all four quads of a thread in flight: t = tid + 256 h
     {
-        constexpr int QPT = 1024 / DM_THREADS;
         cd v[QPT][4];
         S *p0[QPT], *p1[QPT], *p2[QPT], *p3[QPT];
         int kind[QPT];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
@@ -509,14 +512,24 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
                 v[h][0] = ldc(p0[h]);
                 v[h][2] = ldc(p2[h]);
                 v[h][3] = ldc(p3[h]);
-                v[h][1] = kind[h] == 1 ? ldc(p1[h])
-                        : kind[h] == 2 ? cconj(ms[x * LDS_STRIDE + y]) : cconj(v[h][2]);
+                if (kind[h] == 1) v[h][1] = ldc(p1[h]);
             }
+        }
+        if (has_mirror) {
+#pragma unroll
+            for (int h = 0; h < QPT; ++h) {
+                const int t = tid + DM_THREADS * h, x = t >> 5, y = t & 31;
+                const u64 a = A * 32 + x, b = Bk * 32 + y;
+                if (a > b && (a >> q) == (b >> q)) ms[x * LDS_STRIDE + y] = mv[h];
+            }
+            __syncthreads();
         }
 #pragma unroll
         for (int h = 0; h < QPT; ++h) {
             if (!kind[h]) continue;
             const int tt = tid + DM_THREADS * h, x = tt & 31, y = tt >> 5;
+            if (kind[h] == 2) v[h][1] = cconj(ms[x * LDS_STRIDE + y]);
+            else if (kind[h] == 3) v[h][1] = cconj(v[h][2]);
             cd o0, o1, o2, o3;
             sop4(op, v[h][0], v[h][1], v[h][2], v[h][3], o0, o1, o2, o3);
             stc(p0[h], o0);
