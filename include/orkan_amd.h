@@ -82,8 +82,17 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups,
  * 9: streamed packed superoperator, 10: statevector mean / matel_diag reduction (lds_bytes only),
  * 11: surrogate GEMM tile where the shape allows (flags only: 0 128 x 128, 1 256 x 256 staged
- * through registers, 2 256 x 256 streamed into LDS by DMA, the default));
- * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column;
+ * through registers, 2 256 x 256 streamed into LDS by DMA, the default), 12: statevector 1-qubit
+ * kernels for every target qubit (flags bit 4 on: bit 0 XCD remap, bit 1 nontemporal loads and
+ * stores, bits 2-3 log2 of the pairs per thread, 0..2; bit 4 off: back to the tuned per-qubit table),
+ * 13: nontemporal (streaming) accesses, used on states of at least lds_bytes (0: 512 MiB, the
+ * default) by the statevector kernels and the streamed tiled density-matrix kernels; flags switch
+ * the statevector ones off per family (bit 0 1-qubit gates, bit 1 cx / cz / swap / ccx ..., bit 2
+ * exp_diag, bit 3 mean / matel_diag; 0, the default, leaves them on where the tuned shapes use
+ * them), 14: every
+ * launch shape and tuning choice above back to the built-in defaults (flags and lds_bytes unused));
+ * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
+ * (streamed kernels 6, 7, 9) uses nontemporal loads and stores;
  * lds_bytes caps workgroups per CU (0: no cap) */
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);
 

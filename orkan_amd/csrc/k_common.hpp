@@ -41,6 +41,14 @@ template <typename S> struct Store;
 template <> struct Store<double2> {
     static __device__ __forceinline__ cd ld(const double2 *p) { return *p; }
     static __device__ __forceinline__ void st(double2 *p, cd v) { *p = v; }
+    // nontemporal (streaming) forms: the two halves merge into one 16 B access with the nt bit
+    static __device__ __forceinline__ cd ldn(const double2 *p) {
+        return cmk(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+    }
+    static __device__ __forceinline__ void stn(double2 *p, cd v) {
+        __builtin_nontemporal_store(v.x, &p->x);
+        __builtin_nontemporal_store(v.y, &p->y);
+    }
 };
 template <> struct Store<float2> {
     static __device__ __forceinline__ cd ld(const float2 *p) {
@@ -53,9 +61,25 @@ template <> struct Store<float2> {
         f.y = (float)v.y;
         *p = f;
     }
+    static __device__ __forceinline__ cd ldn(const float2 *p) {
+        return cmk((double)__builtin_nontemporal_load(&p->x), (double)__builtin_nontemporal_load(&p->y));
+    }
+    static __device__ __forceinline__ void stn(float2 *p, cd v) {
+        __builtin_nontemporal_store((float)v.x, &p->x);
+        __builtin_nontemporal_store((float)v.y, &p->y);
+    }
 };
 template <typename S> __device__ __forceinline__ cd ldc(const S *p) { return Store<S>::ld(p); }
 template <typename S> __device__ __forceinline__ void stc(S *p, cd v) { Store<S>::st(p, v); }
+// NT selects the nontemporal forms
+template <int NT, typename S> __device__ __forceinline__ cd ldx(const S *p) {
+    if constexpr (NT) return Store<S>::ldn(p);
+    else return Store<S>::ld(p);
+}
+template <int NT, typename S> __device__ __forceinline__ void stx(S *p, cd v) {
+    if constexpr (NT) Store<S>::stn(p, v);
+    else Store<S>::st(p, v);
+}
 
 // ---- index enumeration -------------------------------------------------------------------
 // insert a 0 bit at position `pos`

@@ -307,7 +307,7 @@ __device__ __forceinline__ void sop4(const DmOp &op, cd v0, cd v1, cd v2, cd v3,
 
 __device__ __forceinline__ u64 tile_base(u64 TR, u64 TC, int L) { return (TR * (TR + 1) / 2 + TC) << (2 * L); }
 
-template <typename S, int U>
+template <typename S, int U, int NT>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     const int L = sp.L, q = sp.m, hl = L - 1;
     const int mb = 1 << q;
@@ -325,10 +325,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
         t[u] = ok[u] ? rho + ((Q >> (2 * hl)) << (2 * L)) : rho;
         a00[u] = (r0 << L) | c0;
         if (ok[u]) {
-            v[u][0] = ldc(t[u] + a00[u]);
-            v[u][1] = ldc(t[u] + (a00[u] | mb));
-            v[u][2] = ldc(t[u] + (a00[u] | (mb << L)));
-            v[u][3] = ldc(t[u] + (a00[u] | (mb << L) | mb));
+            v[u][0] = ldx<NT>(t[u] + a00[u]);
+            v[u][1] = ldx<NT>(t[u] + (a00[u] | mb));
+            v[u][2] = ldx<NT>(t[u] + (a00[u] | (mb << L)));
+            v[u][3] = ldx<NT>(t[u] + (a00[u] | (mb << L) | mb));
         }
     }
 #pragma unroll
@@ -336,14 +336,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
         if (!ok[u]) continue;
         cd o0, o1, o2, o3;
         sop4(op, v[u][0], v[u][1], v[u][2], v[u][3], o0, o1, o2, o3);
-        stc(t[u] + a00[u], o0);
-        stc(t[u] + (a00[u] | mb), o1);
-        stc(t[u] + (a00[u] | (mb << L)), o2);
-        stc(t[u] + (a00[u] | (mb << L) | mb), o3);
+        stx<NT>(t[u] + a00[u], o0);
+        stx<NT>(t[u] + (a00[u] | mb), o1);
+        stx<NT>(t[u] + (a00[u] | (mb << L)), o2);
+        stx<NT>(t[u] + (a00[u] | (mb << L) | mb), o3);
     }
 }
 
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     __shared__ cd xs[W_MAX * LDS_STRIDE];
     const int L = sp.L, T = 1 << L, TT = T * T, tid = threadIdx.x;
@@ -365,29 +365,29 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int o = tid + e * DM_THREADS;
-                va[e] = ldc(A + o); vb[e] = ldc(B + o); vc[e] = ldc(C + o); vd[e] = ldc(D + o);
+                va[e] = ldx<NT>(A + o); vb[e] = ldx<NT>(B + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o);
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int o = tid + e * DM_THREADS;
                 cd p0, p1, p2, p3;
                 sop4(op, va[e], vb[e], vc[e], vd[e], p0, p1, p2, p3);
-                stc(A + o, p0); stc(B + o, p1); stc(C + o, p2); stc(D + o, p3);
+                stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
             }
             return;
         }
         for (int o = tid; o < TT; o += 2 * DM_THREADS) {
             const int o2 = o + DM_THREADS;
             const bool two = o2 < TT;
-            const cd a0 = ldc(A + o), b0 = ldc(B + o), c0 = ldc(C + o), d0 = ldc(D + o);
+            const cd a0 = ldx<NT>(A + o), b0 = ldx<NT>(B + o), c0 = ldx<NT>(C + o), d0 = ldx<NT>(D + o);
             cd a1 = a0, b1 = b0, c1 = c0, d1 = d0;
-            if (two) { a1 = ldc(A + o2); b1 = ldc(B + o2); c1 = ldc(C + o2); d1 = ldc(D + o2); }
+            if (two) { a1 = ldx<NT>(A + o2); b1 = ldx<NT>(B + o2); c1 = ldx<NT>(C + o2); d1 = ldx<NT>(D + o2); }
             cd p0, p1, p2, p3;
             sop4(op, a0, b0, c0, d0, p0, p1, p2, p3);
-            stc(A + o, p0); stc(B + o, p1); stc(C + o, p2); stc(D + o, p3);
+            stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
             if (two) {
                 sop4(op, a1, b1, c1, d1, p0, p1, p2, p3);
-                stc(A + o2, p0); stc(B + o2, p1); stc(C + o2, p2); stc(D + o2, p3);
+                stx<NT>(A + o2, p0); stx<NT>(B + o2, p1); stx<NT>(C + o2, p2); stx<NT>(D + o2, p3);
             }
         }
         return;
@@ -402,8 +402,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int o = tid + e * DM_THREADS;
-            va[e] = ldc(A + o); vd[e] = ldc(D + o); vx[e] = ldc(X + o);
-            if (!diag) vc[e] = ldc(C + o);
+            va[e] = ldx<NT>(A + o); vd[e] = ldx<NT>(D + o); vx[e] = ldx<NT>(X + o);
+            if (!diag) vc[e] = ldx<NT>(C + o);
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -419,9 +419,9 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             const cd v2 = diag ? vx[e] : vc[e];
             cd o0, o1, o2, o3;
             sop4(op, va[e], v1, v2, vd[e], o0, o1, o2, o3);
-            stc(A + o, o0);
-            stc(C + o, o2);
-            stc(D + o, o3);
+            stx<NT>(A + o, o0);
+            stx<NT>(C + o, o2);
+            stx<NT>(D + o, o3);
             if (!diag) xs[j * LDS_STRIDE + i] = cconj(o1);
         }
         if (!diag) {
@@ -429,28 +429,28 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int o = tid + e * DM_THREADS;
-                stc(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+                stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
             }
         }
         return;
     }
-    for (int o = tid; o < TT; o += DM_THREADS) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldc(X + o);
+    for (int o = tid; o < TT; o += DM_THREADS) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldx<NT>(X + o);
     __syncthreads();
     for (int o = tid; o < TT; o += DM_THREADS) {
         const int i = o >> L, j = o & (T - 1);
-        const cd v0 = ldc(A + o), v3 = ldc(D + o);
+        const cd v0 = ldx<NT>(A + o), v3 = ldx<NT>(D + o);
         const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
-        const cd v2 = diag ? xs[i * LDS_STRIDE + j] : ldc(C + o);
+        const cd v2 = diag ? xs[i * LDS_STRIDE + j] : ldx<NT>(C + o);
         cd o0, o1, o2, o3;
         sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
-        stc(A + o, o0);
-        stc(C + o, o2);
-        stc(D + o, o3);
+        stx<NT>(A + o, o0);
+        stx<NT>(C + o, o2);
+        stx<NT>(D + o, o3);
         if (!diag) xs[j * LDS_STRIDE + i] = cconj(o1);
     }
     if (!diag) {
         __syncthreads();
-        for (int o = tid; o < TT; o += DM_THREADS) stc(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+        for (int o = tid; o < TT; o += DM_THREADS) stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
     }
 }
 
@@ -470,7 +470,7 @@ struct PackedStreamPlan {
 
 __device__ __forceinline__ u64 col_base(u64 D, u64 c) { return c * (2 * D - c + 1) / 2; }   // offset of (c, c)
 
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict__ rho, PackedStreamPlan pp, DmOp op) {
     __shared__ cd ms[32 * LDS_STRIDE];
     const int q = pp.q, tid = threadIdx.x;
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             const int t = tid + DM_THREADS * h, x = t >> 5, y = t & 31;   // lanes walk y: rows c1 of column r0(x)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
-                mv[h] = ldc(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
+                mv[h] = ldx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
     }
     // all four quads of a thread in flight: t = tid + 256 h
@@ -509,10 +509,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
                 p2[h] = rho + k0 + (r1 - c0);
                 p3[h] = rho + k1 + (r1 - c1);
                 if (kind[h] == 1) p1[h] = rho + k1 + (r0 - c1);   // (r0, c1) is stored only here
-                v[h][0] = ldc(p0[h]);
-                v[h][2] = ldc(p2[h]);
-                v[h][3] = ldc(p3[h]);
-                if (kind[h] == 1) v[h][1] = ldc(p1[h]);
+                v[h][0] = ldx<NT>(p0[h]);
+                v[h][2] = ldx<NT>(p2[h]);
+                v[h][3] = ldx<NT>(p3[h]);
+                if (kind[h] == 1) v[h][1] = ldx<NT>(p1[h]);
             }
         }
         if (has_mirror) {
@@ -532,10 +532,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             else if (kind[h] == 3) v[h][1] = cconj(v[h][2]);
             cd o0, o1, o2, o3;
             sop4(op, v[h][0], v[h][1], v[h][2], v[h][3], o0, o1, o2, o3);
-            stc(p0[h], o0);
-            stc(p2[h], o2);
-            stc(p3[h], o3);
-            if (kind[h] == 1) stc(p1[h], o1);
+            stx<NT>(p0[h], o0);
+            stx<NT>(p2[h], o2);
+            stx<NT>(p3[h], o3);
+            if (kind[h] == 1) stx<NT>(p1[h], o1);
             else if (kind[h] == 2) ms[x * LDS_STRIDE + y] = cconj(o1);
         }
     }
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             const int x = t >> 5, y = t & 31;
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
-                stc(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
+                stx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
         }
     }
 }
@@ -680,9 +680,11 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 // column order plus the XCD remap lets the groups that share the cache lines at
 // the ends of a column run meet in one L2 (without it 1.33x the algorithmic
 // bytes are fetched); tiled: row order with the remap.
-struct Shape { int xcd, col, lds; };
-Shape g_group_shape[2] = {{1, 1, 0}, {1, 0, 0}};   // [packed, tiled]
-Shape g_phase_shape[2] = {{0, 1, 0}, {0, 0, 0}};
+struct Shape { int xcd, col, lds, nt; };   // nt: nontemporal loads and stores
+constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 0}};   // [packed, tiled]
+constexpr Shape kPhaseShape[2] = {{0, 1, 0, 0}, {0, 0, 0, 0}};
+Shape g_group_shape[2] = {kGroupShape[0], kGroupShape[1]};
+Shape g_phase_shape[2] = {kPhaseShape[0], kPhaseShape[1]};
 
 unsigned grid_for(u64 ngroups) {
     const u64 cap = (u64)1 << 30;
@@ -710,8 +712,12 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // streamed tiled superoperator; shape [q < L, q >= L].  Measured on a 14-qubit density matrix
 // (32 x 32 tiles): in-tile targets stream best like the statevector kernels, with consecutive
 // chunks on one XCD and residency capped by 64 KB of LDS per workgroup (6.1 TB/s against 5.6 TB/s
-// uncapped); tile-quad workgroups best without the remap and with an 80 KB cap (5.4 TB/s).
-Shape g_stream_shape[2] = {{1, 0, 65536}, {0, 0, 81920}};
+// uncapped); tile-quad workgroups best without the remap and with a 96 KB cap.  Nontemporal
+// accesses lift the tile quads from 5.4 to 6.0 TB/s and in-tile targets 3 and 4 from 6.2 to 6.55;
+// below target 3 a quad's two columns share 128 B lines and they cost 1-2 % (so they stay off).
+// The packed kernel's column runs share lines between workgroups: nontemporal there loses 20 %.
+constexpr Shape kStreamShape[2] = {{1, 0, 65536, 1}, {0, 0, 98304, 1}};
+Shape g_stream_shape[2] = {kStreamShape[0], kStreamShape[1]};
 int g_stream_on = 1;
 
 template <typename S>
@@ -719,7 +725,8 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     const u64 nt = (u64)1 << (n - L);
     StreamPlan sp;
     sp.L = L;
-    const Shape sh = g_stream_shape[q < L ? 0 : 1];
+    Shape sh = g_stream_shape[q < L ? 0 : 1];
+    if (((nt * (nt + 1) / 2) << (2 * L)) * sizeof(S) < ork_nt_min_bytes()) sh.nt = 0;
     sp.xcd = sh.xcd;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
     if (q < L) {
@@ -729,14 +736,18 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
         sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 2);
         sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        dm_sop_intile_kernel<S, U><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
+        if (sh.nt && q >= 3)
+            dm_sop_intile_kernel<S, U, 1><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
+        else
+            dm_sop_intile_kernel<S, U, 0><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
     } else {
         sp.m = q - L;
         const u64 h = nt >> 1;
         sp.nitems = h * (h + 1) / 2;
         sp.nblocks = sp.nitems;
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        dm_sop_tiles_kernel<S><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
+        if (sh.nt) dm_sop_tiles_kernel<S, 1><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
+        else dm_sop_tiles_kernel<S, 0><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
     }
     ORK_LAUNCH_CHECK();
     return 0;
@@ -745,7 +756,8 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
 // streamed packed superoperator: column order with the XCD remap (blocks that continue the same
 // column runs meet in one L2) and an 80 KB residency cap measured best on 14 qubits (5.0 TB/s
 // against 4.7 TB/s for the tile groups)
-Shape g_pstream_shape = {1, 1, 81920};
+constexpr Shape kPstreamShape = {1, 1, 81920, 0};
+Shape g_pstream_shape = kPstreamShape;
 
 template <typename S>
 int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
@@ -759,7 +771,8 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     if (pp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
     const int lds = g_pstream_shape.lds > (int)sizeof(cd) * 32 * LDS_STRIDE
                         ? g_pstream_shape.lds - (int)sizeof(cd) * 32 * LDS_STRIDE : 0;
-    dm_sop_packed_kernel<S><<<(unsigned)pp.nblocks, DM_THREADS, lds, st>>>((S *)rho, pp, op);
+    if (g_pstream_shape.nt) dm_sop_packed_kernel<S, 1><<<(unsigned)pp.nblocks, DM_THREADS, lds, st>>>((S *)rho, pp, op);
+    else dm_sop_packed_kernel<S, 0><<<(unsigned)pp.nblocks, DM_THREADS, lds, st>>>((S *)rho, pp, op);
     ORK_LAUNCH_CHECK();
     return 0;
 }
@@ -795,9 +808,19 @@ DmPlan tile_plan(int layout, int L, int n) {
 // 3 group kernel tiled, 4 phase kernel packed, 5 phase kernel tiled, 6 / 7 streamed tiled
 // superoperator with the target qubit inside / above the tile, 8 streamed superoperators on (lds_bytes
 // != 0) or off, 9 streamed packed superoperator; flags bit 0: XCD remap, bit 1: column order of groups
+extern "C" void ork_dm_reset_shapes(void) {
+    for (int i = 0; i < 2; ++i) {
+        g_group_shape[i] = kGroupShape[i];
+        g_phase_shape[i] = kPhaseShape[i];
+        g_stream_shape[i] = kStreamShape[i];
+    }
+    g_stream_on = 1;
+    g_pstream_shape = kPstreamShape;
+}
+
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 3) return (int)hipErrorInvalidValue;
-    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes};
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 7) return (int)hipErrorInvalidValue;
+    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1};
     switch (which) {
     case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
     case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;

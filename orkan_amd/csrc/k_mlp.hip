@@ -597,7 +597,10 @@ unsigned g1(long n) {
 }  // namespace
 
 // ---- C ABI ------------------------------------------------------------------------------------
-static int g_gemm_big = 2;   // 256 x 256 tiles where the shape allows: 1 register-staged, 2 DMA-staged
+constexpr int kGemmVariant = 2;
+static int g_gemm_big = kGemmVariant;   // 256 x 256 tiles where the shape allows: 1 register-staged, 2 DMA-staged
+extern "C" void ork_mlp_gemm_reset(void) { g_gemm_big = kGemmVariant; }
+
 extern "C" int ork_mlp_gemm_select(int variant) {
     if (variant < 0 || variant > 2) return (int)hipErrorInvalidValue;
     g_gemm_big = variant;

@@ -85,18 +85,35 @@ struct SvShape {
     int xcd;   // remap block ids so consecutive chunks run on the same XCD
     int u;     // pairs per thread (1 or 2; doubled for complex64 storage)
     int lds;   // dynamic LDS bytes per workgroup (0: no cap)
+    int nt;    // nontemporal loads and stores
 };
 
-// best shape per target qubit of a 2^28-amplitude state (sweep of xcd x u x LDS cap)
-static const int kPairShapeHost[28][3] = {
-    {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248}, {1, 2, 65536}, {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248},
-    {0, 1, 81920}, {1, 1, 53248}, {1, 2, 81920}, {1, 2, 65536}, {1, 2, 81920}, {1, 2, 81920}, {1, 2, 65536},
-    {0, 1, 81920}, {0, 1, 81920}, {0, 1, 65536}, {1, 2, 81920}, {1, 2, 81920}, {1, 1, 81920}, {1, 1, 40960},
-    {1, 1, 81920}, {0, 1, 65536}, {0, 1, 65536}, {0, 1, 65536}, {1, 1, 53248}, {1, 1, 53248}, {1, 1, 53248}};
+// best shape per target qubit of a 2^28-amplitude state (sweep of xcd x u x LDS cap x nontemporal).
+// Nontemporal accesses gain 6-10 % from target 3 up; below that a pair shares its 128 B line with
+// the next pair, which the cache then has to supply twice.
+static const int kPairShapeHost[28][4] = {
+    {1, 1, 53248, 0}, {1, 1, 53248, 0}, {1, 1, 53248, 0}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1},
+    {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1},
+    {1, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1},
+    {1, 1, 40960, 1}, {1, 1, 40960, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 40960, 1}, {0, 1, 53248, 1},
+    {0, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 40960, 1}, {1, 1, 40960, 1}};
+
+// tuning override of the table for every target qubit (used when g_pair_override.u != 0)
+static SvShape g_pair_override{0, 0, 0, 0};
+// nontemporal accesses switched off per kernel family (tuning hook): bit 0 1-qubit pair kernels,
+// bit 1 permutations (cx ...), bit 2 exp_diag, bit 3 reductions
+static int g_nt_off = 0;
+// states below this size may stay in the 256 MB last-level cache between gates, where streaming
+// (cache-bypassing) accesses would only evict them
+constexpr u64 kNtMinBytes = (u64)512 << 20;
+static u64 g_nt_min_bytes = kNtMinBytes;
 
 SvShape pair_shape(int q) {
-    SvShape s{1, 1, 53248};
-    if (q >= 0 && q < 28) s = SvShape{kPairShapeHost[q][0], kPairShapeHost[q][1], kPairShapeHost[q][2]};
+    if (g_pair_override.u) return g_pair_override;
+    SvShape s{1, 1, 53248, 0};
+    if (q >= 0 && q < 28)
+        s = SvShape{kPairShapeHost[q][0], kPairShapeHost[q][1], kPairShapeHost[q][2], kPairShapeHost[q][3]};
+    if (g_nt_off & 1) s.nt = 0;
     return s;
 }
 
@@ -111,7 +128,7 @@ __device__ __forceinline__ u64 chunk_of(int xcd) {
     return b;
 }
 
-template <typename S, int OP, int U, int XCD>
+template <typename S, int OP, int U, int XCD, int NT>
 __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
     const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
     const u64 stride = (u64)1 << q;
@@ -122,8 +139,8 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
         const u64 kk = base + (u64)u * SV_THREADS;
         i0[u] = ins0(kk, q);
         if (kk < npairs) {
-            a[u] = ldc(psi + i0[u]);
-            b[u] = ldc(psi + i0[u] + stride);
+            a[u] = ldx<NT>(psi + i0[u]);
+            b[u] = ldx<NT>(psi + i0[u] + stride);
         }
     }
 #pragma unroll
@@ -131,13 +148,13 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
         const u64 kk = base + (u64)u * SV_THREADS;
         if (kk < npairs) {
             pair_op<OP>(a[u], b[u], k);
-            stc(psi + i0[u], a[u]);
-            stc(psi + i0[u] + stride, b[u]);
+            stx<NT>(psi + i0[u], a[u]);
+            stx<NT>(psi + i0[u] + stride, b[u]);
         }
     }
 }
 
-template <typename S, int OP, int U, int XCD>
+template <typename S, int OP, int U, int XCD, int NT>
 __global__ __launch_bounds__(SV_THREADS) void sv_one_kernel(S *__restrict__ psi, u64 npairs, int q, Coef k) {
     const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
     const u64 stride = (u64)1 << q;
@@ -147,12 +164,12 @@ __global__ __launch_bounds__(SV_THREADS) void sv_one_kernel(S *__restrict__ psi,
     for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
         i1[u] = ins0(kk, q) + stride;
-        if (kk < npairs) b[u] = ldc(psi + i1[u]);
+        if (kk < npairs) b[u] = ldx<NT>(psi + i1[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const u64 kk = base + (u64)u * SV_THREADS;
-        if (kk < npairs) stc(psi + i1[u], one_op<OP>(b[u], k));
+        if (kk < npairs) stx<NT>(psi + i1[u], one_op<OP>(b[u], k));
     }
 }
 
@@ -165,7 +182,7 @@ struct MonoArgs {
     int ph[8];        // times i^ph
 };
 
-template <typename S, int U>
+template <typename S, int U, int NT>
 __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi, u64 nbase, MonoArgs m) {
     const u64 b0 = chunk_of(1) * (SV_THREADS * U) + threadIdx.x;
     // fixed-bound loops with guards keep v[][] in registers (no dynamic indexing)
@@ -178,7 +195,7 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
         if (b < nbase) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (j < m.nt) v[u][j] = ldc(psi + (base[u] | m.src[j]));
+                if (j < m.nt) v[u][j] = ldx<NT>(psi + (base[u] | m.src[j]));
         }
     }
 #pragma unroll
@@ -187,21 +204,22 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
         if (b < nbase) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (j < m.nt) stc(psi + (base[u] | m.dst[j]), cipow(v[u][j], m.ph[j]));
+                if (j < m.nt) stx<NT>(psi + (base[u] | m.dst[j]), cipow(v[u][j], m.ph[j]));
         }
     }
 }
 
 // ---- exp_diag: psi_x *= exp(-i diag[x] t) ------------------------------------------------
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(SV_THREADS) void sv_exp_diag_kernel(S *__restrict__ psi, const double *__restrict__ diag,
                                                                  u64 dim, double t) {
     const u64 x = (u64)blockIdx.x * SV_THREADS + threadIdx.x;
     if (x >= dim) return;
     double s, c;
-    sincos(diag[x] * t, &s, &c);
-    const cd v = ldc(psi + x);
-    stc(psi + x, cmk(c * v.x + s * v.y, c * v.y - s * v.x));
+    const double d = NT ? __builtin_nontemporal_load(diag + x) : diag[x];
+    sincos(d * t, &s, &c);
+    const cd v = ldx<NT>(psi + x);
+    stx<NT>(psi + x, cmk(c * v.x + s * v.y, c * v.y - s * v.x));
 }
 
 // ---- reductions ---------------------------------------------------------------------------
@@ -236,7 +254,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // together, so each thread keeps four loads in flight; they are added in that fixed order.
 constexpr int RED_UNROLL = 4;
 
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restrict__ psi, const double *__restrict__ obs,
                                                               u64 dim, double *__restrict__ part) {
     double a4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0};
@@ -248,8 +266,8 @@ __global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restric
         for (int u = 0; u < RED_UNROLL; ++u) {
             const u64 x = x0 + (u64)u * G;
             const bool in = x < dim;
-            a[u] = in ? ldc(psi + x) : cmk(0.0, 0.0);
-            o[u] = in ? obs[x] : 0.0;
+            a[u] = in ? ldx<NT>(psi + x) : cmk(0.0, 0.0);
+            o[u] = in ? (NT ? __builtin_nontemporal_load(obs + x) : obs[x]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < RED_UNROLL; ++u) a4[u] += o[u] * (a[u].x * a[u].x + a[u].y * a[u].y);
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restric
     if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restrict__ bra, const S *__restrict__ ket,
                                                                const double *__restrict__ obs, u64 dim,
                                                                double *__restrict__ part) {
@@ -272,9 +290,9 @@ __global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restri
         for (int u = 0; u < RED_UNROLL; ++u) {
             const u64 x = x0 + (u64)u * G;
             const bool in = x < dim;
-            a[u] = in ? ldc(bra + x) : cmk(0.0, 0.0);
-            b[u] = in ? ldc(ket + x) : cmk(0.0, 0.0);
-            o[u] = in ? obs[x] : 0.0;
+            a[u] = in ? ldx<NT>(bra + x) : cmk(0.0, 0.0);
+            b[u] = in ? ldx<NT>(ket + x) : cmk(0.0, 0.0);
+            o[u] = in ? (NT ? __builtin_nontemporal_load(obs + x) : obs[x]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < RED_UNROLL; ++u) {
@@ -305,15 +323,21 @@ __global__ __launch_bounds__(RED_THREADS) void finish_kernel(double *__restrict_
         for (int i = 0; i < NV; ++i) part[NV * RED_BLOCKS + i] = acc[i];
 }
 
-template <typename S, int OP, int U, int XCD>
+template <typename S, int OP, int U, int XCD, int NT>
 void launch_shape(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, int lds) {
     const u64 per = (u64)SV_THREADS * U;
     const unsigned grid = (unsigned)((npairs + per - 1) / per);
     if (OP == ORK_SV_P || OP == ORK_SV_Z || OP == ORK_SV_S || OP == ORK_SV_SDG || OP == ORK_SV_T ||
         OP == ORK_SV_TDG)
-        sv_one_kernel<S, OP, U, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
+        sv_one_kernel<S, OP, U, XCD, NT><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
     else
-        sv_pair_kernel<S, OP, U, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
+        sv_pair_kernel<S, OP, U, XCD, NT><<<grid, SV_THREADS, lds, st>>>((S *)psi, npairs, q, k);
+}
+
+template <typename S, int OP, int U, int XCD>
+void launch_nt(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, int lds, int nt) {
+    if (nt) launch_shape<S, OP, U, XCD, 1>(st, psi, npairs, q, k, lds);
+    else launch_shape<S, OP, U, XCD, 0>(st, psi, npairs, q, k, lds);
 }
 
 template <typename S, int OP>
@@ -327,15 +351,16 @@ void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
     }
     // the remap needs a grid of at least 8 blocks; small states use the plain order
     if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
+    if (2 * npairs * sizeof(S) < g_nt_min_bytes) sh.nt = 0;
     if (sh.u == 4) {
-        if (sh.xcd) launch_shape<S, OP, 4, 1>(st, psi, npairs, q, k, sh.lds);
-        else launch_shape<S, OP, 4, 0>(st, psi, npairs, q, k, sh.lds);
+        if (sh.xcd) launch_nt<S, OP, 4, 1>(st, psi, npairs, q, k, sh.lds, sh.nt);
+        else launch_nt<S, OP, 4, 0>(st, psi, npairs, q, k, sh.lds, sh.nt);
     } else if (sh.u == 2) {
-        if (sh.xcd) launch_shape<S, OP, 2, 1>(st, psi, npairs, q, k, sh.lds);
-        else launch_shape<S, OP, 2, 0>(st, psi, npairs, q, k, sh.lds);
+        if (sh.xcd) launch_nt<S, OP, 2, 1>(st, psi, npairs, q, k, sh.lds, sh.nt);
+        else launch_nt<S, OP, 2, 0>(st, psi, npairs, q, k, sh.lds, sh.nt);
     } else {
-        if (sh.xcd) launch_shape<S, OP, 1, 1>(st, psi, npairs, q, k, sh.lds);
-        else launch_shape<S, OP, 1, 0>(st, psi, npairs, q, k, sh.lds);
+        if (sh.xcd) launch_nt<S, OP, 1, 1>(st, psi, npairs, q, k, sh.lds, sh.nt);
+        else launch_nt<S, OP, 1, 0>(st, psi, npairs, q, k, sh.lds, sh.nt);
     }
 }
 
@@ -401,14 +426,19 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     const u64 nbase = ((u64)1 << n) >> m.k;
     hipStream_t st = (hipStream_t)stream;
     const int lds = 53248;   // same residency cap as the pair kernel
+    // streaming accesses when no gate qubit lies inside a 128 B line (as for the pair kernel)
+    const u64 bytes = ((u64)1 << n) * (prec == ORK_F32 ? 8 : 16);
+    const bool nt = !(g_nt_off & 2) && m.pos[0] >= 3 && bytes >= g_nt_min_bytes;
     if (prec == ORK_F32) {
         const u64 per = (u64)SV_THREADS * SV_UNROLL * 2;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
-        sv_mono_kernel<float2, SV_UNROLL * 2><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
+        if (nt) sv_mono_kernel<float2, SV_UNROLL * 2, 1><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
+        else sv_mono_kernel<float2, SV_UNROLL * 2, 0><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
     } else {
         const u64 per = (u64)SV_THREADS * SV_UNROLL;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
-        sv_mono_kernel<double2, SV_UNROLL><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
+        if (nt) sv_mono_kernel<double2, SV_UNROLL, 1><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
+        else sv_mono_kernel<double2, SV_UNROLL, 0><<<grid, SV_THREADS, lds, st>>>((double2 *)psi, nbase, m);
     }
     ORK_LAUNCH_CHECK();
     return 0;
@@ -418,14 +448,48 @@ extern "C" int ork_sv_exp_diag(void *stream, int prec, void *psi, int n, const d
     const u64 dim = (u64)1 << n;
     const unsigned grid = (unsigned)((dim + SV_THREADS - 1) / SV_THREADS);
     hipStream_t st = (hipStream_t)stream;
-    if (prec == ORK_F32) sv_exp_diag_kernel<float2><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, diag, dim, t);
-    else sv_exp_diag_kernel<double2><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, diag, dim, t);
+    const bool nt = !(g_nt_off & 4) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
+    if (prec == ORK_F32) {
+        if (nt) sv_exp_diag_kernel<float2, 1><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, diag, dim, t);
+        else sv_exp_diag_kernel<float2, 0><<<grid, SV_THREADS, 0, st>>>((float2 *)psi, diag, dim, t);
+    } else {
+        if (nt) sv_exp_diag_kernel<double2, 1><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, diag, dim, t);
+        else sv_exp_diag_kernel<double2, 0><<<grid, SV_THREADS, 0, st>>>((double2 *)psi, diag, dim, t);
+    }
     ORK_LAUNCH_CHECK();
     return 0;
 }
 
 // residency cap (dynamic LDS bytes per workgroup) of the mean / matel first stage
-static int g_red_lds = 81920;   // measured best on 2^28 amplitudes (5.33 TB/s against 5.21 uncapped)
+constexpr int kRedLds = 81920;   // measured best on 2^28 amplitudes (5.33 TB/s against 5.21 uncapped)
+static int g_red_lds = kRedLds;
+// flags bit 4 on: every target qubit uses (bit 0 xcd, bit 1 nontemporal, bits 2-3 log2 of the pairs
+// per thread: 0..2) with LDS cap lds; bit 4 off: back to the per-qubit table
+extern "C" int ork_sv_set_pair_shape(int flags, int lds) {
+    if (lds < 0 || lds > 160 * 1024 || flags < 0 || flags > 31 || ((flags >> 2) & 3) == 3) return (int)hipErrorInvalidValue;
+    if (!(flags & 16)) g_pair_override = SvShape{0, 0, 0, 0};
+    else g_pair_override = SvShape{flags & 1, 1 << ((flags >> 2) & 3), lds, (flags >> 1) & 1};
+    return 0;
+}
+
+// tuning hook: nontemporal accesses off per kernel family (bit 0 pair kernels, bit 1 permutations,
+// bit 2 exp_diag, bit 3 reductions); min_bytes: smallest state that uses them (0: 512 MiB)
+extern "C" int ork_sv_set_nt_off(int mask, int min_bytes) {
+    if (mask < 0 || mask > 15 || min_bytes < 0) return (int)hipErrorInvalidValue;
+    g_nt_off = mask;
+    g_nt_min_bytes = min_bytes ? (u64)min_bytes : kNtMinBytes;
+    return 0;
+}
+
+extern "C" unsigned long long ork_nt_min_bytes(void) { return g_nt_min_bytes; }
+
+extern "C" void ork_sv_reset_tuning(void) {
+    g_pair_override = SvShape{0, 0, 0, 0};
+    g_nt_off = 0;
+    g_nt_min_bytes = kNtMinBytes;
+    g_red_lds = kRedLds;
+}
+
 extern "C" int ork_sv_set_red_lds(int lds) {
     if (lds < 0 || lds > 160 * 1024) return (int)hipErrorInvalidValue;
     g_red_lds = lds;
@@ -444,8 +508,14 @@ extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const
     const u64 dim = (u64)1 << n;
     const int nb = red_blocks(dim);
     hipStream_t st = (hipStream_t)stream;
-    if (prec == ORK_F32) sv_mean_kernel<float2><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)psi, obs, dim, part);
-    else sv_mean_kernel<double2><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)psi, obs, dim, part);
+    const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
+    if (prec == ORK_F32) {
+        if (nt) sv_mean_kernel<float2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)psi, obs, dim, part);
+        else sv_mean_kernel<float2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)psi, obs, dim, part);
+    } else {
+        if (nt) sv_mean_kernel<double2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)psi, obs, dim, part);
+        else sv_mean_kernel<double2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)psi, obs, dim, part);
+    }
     ORK_LAUNCH_CHECK();
     finish_kernel<1><<<1, RED_THREADS, 0, st>>>(part, nb);
     ORK_LAUNCH_CHECK();
@@ -459,12 +529,22 @@ extern "C" int ork_sv_matel(void *stream, int prec, const void *bra, const void 
     const u64 dim = (u64)1 << n;
     const int nb = red_blocks(dim);
     hipStream_t st = (hipStream_t)stream;
-    if (prec == ORK_F32)
-        sv_matel_kernel<float2><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)bra, (const float2 *)ket, obs, dim,
-                                                                    part);
-    else
-        sv_matel_kernel<double2><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)bra, (const double2 *)ket, obs,
-                                                                     dim, part);
+    const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
+    if (prec == ORK_F32) {
+        if (nt)
+            sv_matel_kernel<float2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)bra, (const float2 *)ket, obs,
+                                                                           dim, part);
+        else
+            sv_matel_kernel<float2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)bra, (const float2 *)ket, obs,
+                                                                           dim, part);
+    } else {
+        if (nt)
+            sv_matel_kernel<double2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)bra, (const double2 *)ket,
+                                                                            obs, dim, part);
+        else
+            sv_matel_kernel<double2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)bra, (const double2 *)ket,
+                                                                            obs, dim, part);
+    }
     ORK_LAUNCH_CHECK();
     finish_kernel<2><<<1, RED_THREADS, 0, st>>>(part, nb);
     ORK_LAUNCH_CHECK();

@@ -86,6 +86,12 @@ int ork_dm_set_shape(int which, int flags, int lds_bytes);
 /* residency cap (dynamic LDS bytes per workgroup) of the statevector mean / matel reduction */
 int ork_sv_set_red_lds(int lds_bytes);
 int ork_mlp_gemm_select(int variant);
+int ork_sv_set_pair_shape(int flags, int lds_bytes);
+int ork_sv_set_nt_off(int mask, int min_bytes);
+unsigned long long ork_nt_min_bytes(void);
+void ork_dm_reset_shapes(void);
+void ork_sv_reset_tuning(void);
+void ork_mlp_gemm_reset(void);
 
 /* number of doubles the *_mean / matel launchers need in partials_dev */
 uint64_t ork_partials_len(void);

@@ -1,0 +1,115 @@
+"""Nontemporal (streaming) loads and stores change only the cache policy of an
+access, never a value: every kernel family that has them gives bit-identical
+results with them on and off.
+
+Statevector kernels use them on states of 512 MiB and more (2^25 amplitudes in
+complex128, 2^26 in complex64), where the tuned shapes switch them on; the hook
+(kernel 13) turns them off per family and sets that size.  The streamed
+density-matrix kernels take them from bit 2 of their launch-shape flags
+(kernels 6, 7, 9) on states of at least that size, which the test lowers to
+one byte for a 10-qubit density matrix."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import engines
+import orkan_amd
+from linalg_np import MIXED_PACKED, MIXED_TILED, PURE, to_layout
+from orkan_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+NT_OFF_HOOK, RESET_HOOK = 13, 14
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = engines.DeviceEngine(orkan_amd.lib_path(5), 5)
+    yield e
+    assert e.api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+
+
+def _sv_run(eng, n, fp32, off, psi, obs_dev):
+    api = eng.api
+    assert api.orkan_set_launch_shape(NT_OFF_HOOK, off, 0) == 0
+    st = eng.new(PURE, n, psi)
+    if fp32:
+        st32 = api.orkan_state_convert(C.byref(st), 1)
+        eng.free(st)
+        st = st32
+    ps = C.byref(st)
+    obs_p = C.cast(obs_dev, C.POINTER(C.c_double))
+    for q in (0, 2, 3, 10, n - 1):
+        api.rx(ps, q, 0.3 + 0.1 * q)
+        api.ry(ps, q, 0.7)
+        api.rz(ps, q, 1.1)
+        api.h(ps, q)
+    for c, t in ((0, 1), (3, 4), (10, 2), (n - 1, 5)):
+        api.cx(ps, c, t)
+    api.cz(ps, 4, 9)
+    api.exp_diag(ps, obs_p, 0.4)
+    m = api.mean(ps, obs_p)
+    other = api.state_cp(ps)
+    api.rx(C.byref(other), 7, 0.9)
+    me = api.matel_diag(ps, C.byref(other), obs_p)
+    eng.free(other)
+    if fp32:
+        back = api.orkan_state_convert(ps, 0)
+        eng.free(st)
+        st = back
+    out = eng.read(st)
+    eng.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    return out, m, (me.re, me.im)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_statevector_streaming_accesses_change_no_value(eng, fp32):
+    n = 26 if fp32 else 25            # 512 MiB either way
+    g = wl.rng(41)
+    psi = (g.standard_normal(1 << n) + 1j * g.standard_normal(1 << n)) / np.sqrt(2 << n)
+    obs = g.uniform(-1, 1, 1 << n)
+    obs_dev = eng.api.orkan_dev_alloc(1 << n)
+    assert eng.api.orkan_dev_upload(obs_dev, obs.ctypes.data, 1 << n) == 0
+    try:
+        a, ma, ea = _sv_run(eng, n, fp32, 0, psi, obs_dev)
+        b, mb, eb = _sv_run(eng, n, fp32, 15, psi, obs_dev)
+    finally:
+        eng.api.orkan_dev_free(obs_dev)
+    assert np.array_equal(a, b)
+    assert ma == mb and ea == eb
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
+    api = eng.api
+    n = 10
+    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(43)), kind, 5)
+    S = wl.superop(wl.kraus_amp_damp(0.05))
+
+    def run(nt):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
+        assert api.orkan_set_launch_shape(6, 1 | 4 * nt, 65536) == 0
+        assert api.orkan_set_launch_shape(7, 4 * nt, 81920) == 0
+        assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
+        st = eng.new(kind, n, rho)
+        if fp32:
+            st32 = api.orkan_state_convert(C.byref(st), 1)
+            eng.free(st)
+            st = st32
+        for q in range(n):
+            eng.gate(st, "h", q)
+            eng.gate(st, "ry", q, 0.3 + 0.1 * q)
+            eng.channel(st, S, q)
+        if fp32:
+            back = api.orkan_state_convert(C.byref(st), 0)
+            eng.free(st)
+            st = back
+        out = eng.read(st)
+        eng.free(st)
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        return out
+
+    assert np.array_equal(run(1), run(0))
