@@ -714,7 +714,8 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // chunks on one XCD and residency capped by 64 KB of LDS per workgroup (6.1 TB/s against 5.6 TB/s
 // uncapped); tile-quad workgroups best without the remap and with a 96 KB cap.  Nontemporal
 // accesses lift the tile quads from 5.4 to 6.0 TB/s and in-tile targets 3 and 4 from 6.2 to 6.55;
-// below target 3 a quad's two columns share 128 B lines and they cost 1-2 % (so they stay off).
+// where a quad's two columns share 128 B lines (targets below 3 in complex128, below 4 in
+// complex64) they cost 1-2 % and refetch lines (complex64 target 3: 1.1 x the reads), so stay off.
 // The packed kernel's column runs share lines between workgroups: nontemporal there loses 20 %.
 constexpr Shape kStreamShape[2] = {{1, 0, 65536, 1}, {0, 0, 98304, 1}};
 Shape g_stream_shape[2] = {kStreamShape[0], kStreamShape[1]};
@@ -736,7 +737,7 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
         sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 2);
         sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        if (sh.nt && q >= 3)
+        if (sh.nt && ((u64)sizeof(S) << q) >= 128)   // the quad's columns in different 128 B lines
             dm_sop_intile_kernel<S, U, 1><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
         else
             dm_sop_intile_kernel<S, U, 0><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);

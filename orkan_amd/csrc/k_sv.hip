@@ -351,7 +351,9 @@ void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
     }
     // the remap needs a grid of at least 8 blocks; small states use the plain order
     if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
-    if (2 * npairs * sizeof(S) < g_nt_min_bytes) sh.nt = 0;
+    // nontemporal only on large states, and only where a pair's two amplitudes lie in different
+    // 128 B lines (target 3 up in complex128, 4 up in complex64)
+    if (2 * npairs * sizeof(S) < g_nt_min_bytes || ((u64)sizeof(S) << q) < 128) sh.nt = 0;
     if (sh.u == 4) {
         if (sh.xcd) launch_nt<S, OP, 4, 1>(st, psi, npairs, q, k, sh.lds, sh.nt);
         else launch_nt<S, OP, 4, 0>(st, psi, npairs, q, k, sh.lds, sh.nt);
@@ -427,8 +429,8 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     hipStream_t st = (hipStream_t)stream;
     const int lds = 53248;   // same residency cap as the pair kernel
     // streaming accesses when no gate qubit lies inside a 128 B line (as for the pair kernel)
-    const u64 bytes = ((u64)1 << n) * (prec == ORK_F32 ? 8 : 16);
-    const bool nt = !(g_nt_off & 2) && m.pos[0] >= 3 && bytes >= g_nt_min_bytes;
+    const u64 elem = prec == ORK_F32 ? 8 : 16;
+    const bool nt = !(g_nt_off & 2) && (elem << m.pos[0]) >= 128 && (elem << n) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
         const u64 per = (u64)SV_THREADS * SV_UNROLL * 2;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
