@@ -5,9 +5,9 @@ results with them on and off.
 Statevector kernels use them on states of 512 MiB and more (2^25 amplitudes in
 complex128, 2^26 in complex64), where the tuned shapes switch them on; the hook
 (kernel 13) turns them off per family and sets that size.  The streamed
-density-matrix kernels take them from bit 2 of their launch-shape flags
-(kernels 6, 7, 9) on states of at least that size, which the test lowers to
-one byte for a 10-qubit density matrix."""
+density-matrix kernels and the phase kernels (cz, exp_diag) take them from
+bit 2 of their launch-shape flags (kernels 4-7, 9) on states of at least that
+size, which the test lowers to one byte for a 10-qubit density matrix."""
 import ctypes as C
 
 import numpy as np
@@ -87,6 +87,7 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
     n = 10
     rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(43)), kind, 5)
     S = wl.superop(wl.kraus_amp_damp(0.05))
+    diag = wl.rng(44).uniform(-1, 1, 1 << n)
 
     def run(nt):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
@@ -94,6 +95,8 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
         assert api.orkan_set_launch_shape(6, 1 | 4 * nt, 65536) == 0
         assert api.orkan_set_launch_shape(7, 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
+        assert api.orkan_set_launch_shape(4, 3 | 4 * nt, 0) == 0
+        assert api.orkan_set_launch_shape(5, 4 * nt, 0) == 0
         st = eng.new(kind, n, rho)
         if fp32:
             st32 = api.orkan_state_convert(C.byref(st), 1)
@@ -103,6 +106,9 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
             eng.gate(st, "h", q)
             eng.gate(st, "ry", q, 0.3 + 0.1 * q)
             eng.channel(st, S, q)
+        for q in range(n - 1):
+            eng.gate(st, "cz", q, q + 1)
+        eng.exp_diag(st, diag, 0.7)
         if fp32:
             back = api.orkan_state_convert(C.byref(st), 0)
             eng.free(st)
