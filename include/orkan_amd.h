@@ -84,7 +84,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * 11: surrogate GEMM tile where the shape allows (flags only: 0 128 x 128, 1 256 x 256 staged
  * through registers, 2 256 x 256 streamed into LDS by DMA, the default), 12: statevector 1-qubit
  * kernels for every target qubit (flags bit 4 on: bit 0 XCD remap, bit 1 nontemporal loads and
- * stores, bits 2-3 log2 of the pairs per thread, 0..2; bit 4 off: back to the tuned per-qubit table),
+ * stores, bits 2-3 log2 of the pairs per thread, 0..2, in complex128 terms (complex64 storage runs
+ * twice the pairs, at most 4, under half the LDS cap); bit 4 off: back to the tuned per-qubit tables),
  * 13: nontemporal (streaming) accesses, used on states of at least lds_bytes (0: 512 MiB, the
  * default) by the statevector kernels and the streamed tiled density-matrix kernels; flags switch
  * the statevector ones off per family (bit 0 1-qubit gates, bit 1 cx / cz / swap / ccx ..., bit 2

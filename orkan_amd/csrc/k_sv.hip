@@ -98,6 +98,16 @@ static const int kPairShapeHost[28][4] = {
     {1, 1, 40960, 1}, {1, 1, 40960, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 40960, 1}, {0, 1, 53248, 1},
     {0, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 40960, 1}, {1, 1, 40960, 1}};
 
+// complex64 storage (pairs per thread and LDS cap as launched): twice the pairs and half the cap of the
+// complex128 shape, except where a sweep of complex64 shapes on 2^28 amplitudes found better
+// (targets 0-3 and several above 17: 2 pairs per thread in plain order, 77.7 -> 79.1 % over the 28)
+static const int kPairShapeF32[28][4] = {
+    {0, 2, 40960, 0}, {0, 2, 40960, 0}, {0, 2, 32768, 0}, {0, 2, 32768, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1},
+    {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1},
+    {1, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1},
+    {0, 2, 20480, 1}, {0, 2, 20480, 1}, {0, 2, 20480, 1}, {0, 4, 20480, 1}, {1, 2, 20480, 1}, {1, 2, 20480, 1},
+    {0, 2, 26624, 1}, {0, 2, 20480, 1}, {0, 4, 32768, 1}, {0, 2, 20480, 1}};
+
 // tuning override of the table for every target qubit (used when g_pair_override.u != 0)
 static SvShape g_pair_override{0, 0, 0, 0};
 // nontemporal accesses switched off per kernel family (tuning hook): bit 0 1-qubit pair kernels,
@@ -108,11 +118,22 @@ static int g_nt_off = 0;
 constexpr u64 kNtMinBytes = (u64)512 << 20;
 static u64 g_nt_min_bytes = kNtMinBytes;
 
-SvShape pair_shape(int q) {
-    if (g_pair_override.u) return g_pair_override;
+// launch shape for target q; f32: complex64 storage
+SvShape pair_shape(int q, bool f32) {
     SvShape s{1, 1, 53248, 0};
-    if (q >= 0 && q < 28)
+    if (g_pair_override.u || q < 0 || q >= 28) {
+        if (g_pair_override.u) s = g_pair_override;
+        // complex64 storage moves half the bytes per pair: twice the pairs per thread (4 at most,
+        // the largest instantiated shape) and twice the resident workgroups keep the bytes in flight
+        if (f32) {
+            s.u = s.u < 4 ? 2 * s.u : 4;
+            s.lds /= 2;
+        }
+    } else if (f32) {
+        s = SvShape{kPairShapeF32[q][0], kPairShapeF32[q][1], kPairShapeF32[q][2], kPairShapeF32[q][3]};
+    } else {
         s = SvShape{kPairShapeHost[q][0], kPairShapeHost[q][1], kPairShapeHost[q][2], kPairShapeHost[q][3]};
+    }
     if (g_nt_off & 1) s.nt = 0;
     return s;
 }
@@ -342,13 +363,7 @@ void launch_nt(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, int 
 
 template <typename S, int OP>
 void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
-    SvShape sh = pair_shape(q);
-    // complex64 storage moves half the bytes per pair: twice the pairs per thread and twice the
-    // resident workgroups keep the same bytes in flight per CU as the complex128 shape
-    if (sizeof(S) == 8) {
-        sh.u *= 2;
-        sh.lds /= 2;
-    }
+    SvShape sh = pair_shape(q, sizeof(S) == 8);
     // the remap needs a grid of at least 8 blocks; small states use the plain order
     if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
     // nontemporal only on large states, and only where a pair's two amplitudes lie in different
