@@ -93,7 +93,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * them), 14: every
  * launch shape and tuning choice above back to the built-in defaults (flags and lds_bytes unused));
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
- * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9) uses nontemporal loads and stores;
+ * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9) uses nontemporal loads and stores, bits 3-4
+ * (kernel 7, 32 x 32 tiles) give log2 of the workgroups that share one tile quad (0..2);
  * lds_bytes caps workgroups per CU (0: no cap) */
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);
 

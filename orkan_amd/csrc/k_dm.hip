@@ -343,11 +343,16 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
     }
 }
 
-template <typename S, int NT>
+// PARTS > 1 (32 x 32 tiles only): PARTS consecutive workgroups share one tile quad, each taking
+// 32 / PARTS rows of A, C, D (and of B, or the matching columns of its stored mirror); a diagonal
+// tile quad (TR == TC) stays whole in its first workgroup, since there the mirror is C itself.
+template <typename S, int NT, int PARTS>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     __shared__ cd xs[W_MAX * LDS_STRIDE];
     const int L = sp.L, T = 1 << L, TT = T * T, tid = threadIdx.x;
-    const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
+    const u64 g = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
+    const u64 w = g / PARTS;
+    const int part = (int)(g % PARTS);
     if (w >= sp.nitems) return;
     u64 Rr, Cr;
     tri_decode(w, Rr, Cr);
@@ -361,15 +366,17 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
         // in-tile entries of all four tiles at once (64 KB of a workgroup in flight)
         S *B = rho + tile_base(TR, TC | bb, L);
         if (TT == 4 * DM_THREADS) {
-            cd va[4], vb[4], vc[4], vd[4];
+            constexpr int E = 4 / PARTS;
+            const int o0 = part * (4 * DM_THREADS / PARTS);
+            cd va[E], vb[E], vc[E], vd[E];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int o = tid + e * DM_THREADS;
+            for (int e = 0; e < E; ++e) {
+                const int o = o0 + tid + e * DM_THREADS;
                 va[e] = ldx<NT>(A + o); vb[e] = ldx<NT>(B + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int o = tid + e * DM_THREADS;
+            for (int e = 0; e < E; ++e) {
+                const int o = o0 + tid + e * DM_THREADS;
                 cd p0, p1, p2, p3;
                 sop4(op, va[e], vb[e], vc[e], vd[e], p0, p1, p2, p3);
                 stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
@@ -394,6 +401,45 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     }
     const bool diag = (TR == TC);
     S *X = diag ? C : rho + tile_base(TC | bb, TR, L);
+    if (PARTS > 1 && TT == 4 * DM_THREADS && !diag) {
+        // rows [i0, i0 + R) of A, C, D; the matching columns [i0, i0 + R) of the stored mirror X
+        constexpr int E = 4 / PARTS, R = 32 / PARTS;
+        const int i0 = part * R;
+        cd va[E], vc[E], vd[E], vx[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int o = i0 * 32 + tid + e * DM_THREADS;
+            const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
+            va[e] = ldx<NT>(A + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o);
+            vx[e] = ldx<NT>(X + xj * 32 + xi);
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
+            xs[xj * LDS_STRIDE + xi] = vx[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int o = i0 * 32 + tid + e * DM_THREADS, i = o >> 5, j = o & 31;
+            // (j, i) of the LDS copy is read and then rewritten by this thread alone
+            const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
+            cd o0, o1, o2, o3;
+            sop4(op, va[e], v1, vc[e], vd[e], o0, o1, o2, o3);
+            stx<NT>(A + o, o0);
+            stx<NT>(C + o, o2);
+            stx<NT>(D + o, o3);
+            xs[j * LDS_STRIDE + i] = cconj(o1);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
+            stx<NT>(X + xj * 32 + xi, xs[xj * LDS_STRIDE + xi]);
+        }
+        return;
+    }
+    if (part != 0) return;   // a diagonal tile quad is handled whole by its first workgroup
     if (TT == 4 * DM_THREADS) {
         // 32 x 32 tiles: every load of the workgroup (A, D, X and, off the diagonal, C) in flight at
         // once.  On the diagonal X is C itself, so a thread's own C entry comes from its registers
@@ -680,7 +726,8 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 // column order plus the XCD remap lets the groups that share the cache lines at
 // the ends of a column run meet in one L2 (without it 1.33x the algorithmic
 // bytes are fetched); tiled: row order with the remap.
-struct Shape { int xcd, col, lds, nt; };   // nt: nontemporal loads and stores
+// nt: nontemporal loads and stores; lp: log2 of the workgroups sharing one tile quad (cross-tile kernel)
+struct Shape { int xcd, col, lds, nt, lp; };
 constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 0}};   // [packed, tiled]
 // phase kernels (cz, exp_diag, ...) measured on 14 qubits: packed best in column order, tiled in plain
 // order with nontemporal accesses where the changed entries fill whole 128 B lines (cz from qubits
@@ -715,12 +762,13 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // streamed tiled superoperator; shape [q < L, q >= L].  Measured on a 14-qubit density matrix
 // (32 x 32 tiles): in-tile targets stream best like the statevector kernels, with consecutive
 // chunks on one XCD and residency capped by 64 KB of LDS per workgroup (6.1 TB/s against 5.6 TB/s
-// uncapped); tile-quad workgroups best without the remap and with a 96 KB cap.  Nontemporal
+// uncapped); tile quads best split over 4 workgroups (8 rows of each tile, 16 KB per workgroup), in
+// plain order with a 48 KB cap (6.18 TB/s against 5.94 whole).  Nontemporal
 // accesses lift the tile quads from 5.4 to 6.0 TB/s and in-tile targets 3 and 4 from 6.2 to 6.55;
 // where a quad's two columns share 128 B lines (targets below 3 in complex128, below 4 in
 // complex64) they cost 1-2 % and refetch lines (complex64 target 3: 1.1 x the reads), so stay off.
 // The packed kernel's column runs share lines between workgroups: nontemporal there loses 20 %.
-constexpr Shape kStreamShape[2] = {{1, 0, 65536, 1}, {0, 0, 98304, 1}};
+constexpr Shape kStreamShape[2] = {{1, 0, 65536, 1, 0}, {0, 0, 49152, 1, 2}};
 Shape g_stream_shape[2] = {kStreamShape[0], kStreamShape[1]};
 int g_stream_on = 1;
 
@@ -747,11 +795,21 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     } else {
         sp.m = q - L;
         const u64 h = nt >> 1;
+        const int lp = L == 5 ? sh.lp : 0;   // split tile quads: 32 x 32 tiles only
         sp.nitems = h * (h + 1) / 2;
-        sp.nblocks = sp.nitems;
+        sp.nblocks = sp.nitems << lp;
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        if (sh.nt) dm_sop_tiles_kernel<S, 1><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
-        else dm_sop_tiles_kernel<S, 0><<<(unsigned)sp.nblocks, DM_THREADS, lds, st>>>((S *)rho, sp, op);
+        const unsigned nb = (unsigned)sp.nblocks;
+        S *r = (S *)rho;
+        if (sh.nt) {
+            if (lp == 2) dm_sop_tiles_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else if (lp == 1) dm_sop_tiles_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else dm_sop_tiles_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+        } else {
+            if (lp == 2) dm_sop_tiles_kernel<S, 0, 4><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else if (lp == 1) dm_sop_tiles_kernel<S, 0, 2><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else dm_sop_tiles_kernel<S, 0, 1><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+        }
     }
     ORK_LAUNCH_CHECK();
     return 0;
@@ -829,7 +887,8 @@ DmPlan tile_plan(int layout, int L, int n) {
 // 3 group kernel tiled, 4 phase kernel packed, 5 phase kernel tiled, 6 / 7 streamed tiled
 // superoperator with the target qubit inside / above the tile, 8 streamed superoperators on (lds_bytes
 // != 0) or off, 9 streamed packed superoperator; flags bit 0: XCD remap, bit 1: column order of groups,
-// bit 2: nontemporal accesses (phase and streamed kernels)
+// bit 2: nontemporal accesses (phase and streamed kernels), bits 3-4: log2 of the workgroups per tile
+// quad (kernel 7)
 extern "C" void ork_dm_reset_shapes(void) {
     for (int i = 0; i < 2; ++i) {
         g_group_shape[i] = kGroupShape[i];
@@ -841,8 +900,9 @@ extern "C" void ork_dm_reset_shapes(void) {
 }
 
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 7) return (int)hipErrorInvalidValue;
-    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1};
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 31 || (flags >> 3) > 2)
+        return (int)hipErrorInvalidValue;
+    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, flags >> 3};
     switch (which) {
     case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
     case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;

@@ -119,3 +119,40 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
         return out
 
     assert np.array_equal(run(1), run(0))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_split_tile_quads_change_no_value(eng, fp32):
+    """The cross-tile superoperator kernel with 2 or 4 workgroups per tile quad (bits 3-4 of the
+    kernel-7 shape flags) forms the same products for every entry as with one: bit-identical."""
+    api = eng.api
+    n = 9
+    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(45)), MIXED_TILED, 5)
+    S = wl.superop(wl.kraus_depolarizing(0.05))
+
+    def run(lp, nt):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
+        assert api.orkan_set_launch_shape(7, 8 * lp + 4 * nt, 81920) == 0
+        st = eng.new(MIXED_TILED, n, rho)
+        if fp32:
+            st32 = api.orkan_state_convert(C.byref(st), 1)
+            eng.free(st)
+            st = st32
+        for q in range(5, n):
+            eng.gate(st, "h", q)
+            eng.gate(st, "ry", q, 0.2 + 0.1 * q)
+            eng.channel(st, S, q)
+        if fp32:
+            back = api.orkan_state_convert(C.byref(st), 0)
+            eng.free(st)
+            st = back
+        out = eng.read(st)
+        eng.free(st)
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        return out
+
+    ref = run(0, 0)
+    for lp in (1, 2):
+        for nt in (0, 1):
+            assert np.array_equal(run(lp, nt), ref), (lp, nt)
