@@ -516,35 +516,40 @@ struct PackedStreamPlan {
 
 __device__ __forceinline__ u64 col_base(u64 D, u64 c) { return c * (2 * D - c + 1) / 2; }   // offset of (c, c)
 
-template <typename S, int NT>
+// PARTS consecutive workgroups (after the XCD remap) share one 32 x 32 block, each taking 32 / PARTS
+// of its columns b (their quads and their mirror entries): no entry is touched by two of them.
+template <typename S, int NT, int PARTS>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict__ rho, PackedStreamPlan pp, DmOp op) {
     __shared__ cd ms[32 * LDS_STRIDE];
     const int q = pp.q, tid = threadIdx.x;
     const u64 m = (u64)1 << q, D = pp.dim;
+    const u64 g = remap_group(blockIdx.x, pp.nblocks * PARTS, pp.xcd);
+    const int part = (int)(g % PARTS);
     u64 A, Bk;
-    block_rc(blockIdx.x, pp.nblocks, pp.nt, pp.xcd, pp.col, A, Bk);
+    block_rc(g / PARTS, pp.nblocks, pp.nt, 0, pp.col, A, Bk);
     const bool has_mirror = q >= 5 ? ((A >> (q - 5)) == (Bk >> (q - 5))) : (A == Bk);
-    constexpr int QPT = 1024 / DM_THREADS;
+    constexpr int QPT = 1024 / DM_THREADS / PARTS, YW = 32 / PARTS;
+    const int y0 = part * YW;
     // the block's mirror entries, loaded together with the quads below and staged through LDS
     // once every load is in flight
     cd mv[QPT];
     if (has_mirror) {
 #pragma unroll
         for (int h = 0; h < QPT; ++h) {
-            const int t = tid + DM_THREADS * h, x = t >> 5, y = t & 31;   // lanes walk y: rows c1 of column r0(x)
+            const int t = tid + DM_THREADS * h, x = t / YW, y = y0 + t % YW;   // lanes walk y: rows c1 of column r0(x)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
                 mv[h] = ldx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
     }
-    // all four quads of a thread in flight: t = tid + 256 h
+    // all quads of a thread in flight: t = tid + 256 h
     {
         cd v[QPT][4];
         S *p0[QPT], *p1[QPT], *p2[QPT], *p3[QPT];
         int kind[QPT];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
 #pragma unroll
         for (int h = 0; h < QPT; ++h) {
-            const int tt = tid + DM_THREADS * h, x = tt & 31, y = tt >> 5;   // lanes walk x: rows r0 of column c0(y)
+            const int tt = tid + DM_THREADS * h, x = tt & 31, y = y0 + (tt >> 5);   // lanes walk x: rows r0 of column c0(y)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             kind[h] = a < b ? 0 : a == b ? 3 : ((a >> q) == (b >> q) ? 2 : 1);
             p0[h] = p1[h] = p2[h] = p3[h] = rho;
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
         if (has_mirror) {
 #pragma unroll
             for (int h = 0; h < QPT; ++h) {
-                const int t = tid + DM_THREADS * h, x = t >> 5, y = t & 31;
+                const int t = tid + DM_THREADS * h, x = t / YW, y = y0 + t % YW;
                 const u64 a = A * 32 + x, b = Bk * 32 + y;
                 if (a > b && (a >> q) == (b >> q)) ms[x * LDS_STRIDE + y] = mv[h];
             }
@@ -573,7 +578,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
 #pragma unroll
         for (int h = 0; h < QPT; ++h) {
             if (!kind[h]) continue;
-            const int tt = tid + DM_THREADS * h, x = tt & 31, y = tt >> 5;
+            const int tt = tid + DM_THREADS * h, x = tt & 31, y = y0 + (tt >> 5);
             if (kind[h] == 2) v[h][1] = cconj(ms[x * LDS_STRIDE + y]);
             else if (kind[h] == 3) v[h][1] = cconj(v[h][2]);
             cd o0, o1, o2, o3;
@@ -587,8 +592,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
     }
     if (has_mirror) {
         __syncthreads();
-        for (int t = tid; t < 1024; t += DM_THREADS) {
-            const int x = t >> 5, y = t & 31;
+        for (int t = tid; t < 1024 / PARTS; t += DM_THREADS) {
+            const int x = t / YW, y = y0 + t % YW;
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
                 stx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
@@ -815,10 +820,11 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     return 0;
 }
 
-// streamed packed superoperator: column order with the XCD remap (blocks that continue the same
-// column runs meet in one L2) and an 80 KB residency cap measured best on 14 qubits (5.0 TB/s
-// against 4.7 TB/s for the tile groups)
-constexpr Shape kPstreamShape = {1, 1, 81920, 0};
+// streamed packed superoperator: column order and an 80 KB residency cap; each 32 x 32 block split
+// over 4 workgroups of 8 columns (16 KB each), dispatched in order.  Measured on 14 qubits: 5.24
+// TB/s (whole blocks with the XCD remap 5.06, the tile groups 4.7).  Complex64 storage keeps whole
+// blocks (see launch_pstream).
+constexpr Shape kPstreamShape = {0, 1, 81920, 0, 2};
 Shape g_pstream_shape = kPstreamShape;
 
 template <typename S>
@@ -833,8 +839,25 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     if (pp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
     const int lds = g_pstream_shape.lds > (int)sizeof(cd) * 32 * LDS_STRIDE
                         ? g_pstream_shape.lds - (int)sizeof(cd) * 32 * LDS_STRIDE : 0;
-    if (g_pstream_shape.nt) dm_sop_packed_kernel<S, 1><<<(unsigned)pp.nblocks, DM_THREADS, lds, st>>>((S *)rho, pp, op);
-    else dm_sop_packed_kernel<S, 0><<<(unsigned)pp.nblocks, DM_THREADS, lds, st>>>((S *)rho, pp, op);
+    int lp = g_pstream_shape.lp;
+    if (sizeof(S) == 8 && lp) {
+        // complex64: a quarter block per workgroup leaves too few bytes in flight (3.4 against 3.8
+        // TB/s for the whole layer); whole blocks with the XCD remap
+        lp = 0;
+        pp.xcd = 1;
+    }
+    if ((pp.nblocks << lp) > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    const unsigned nb = (unsigned)(pp.nblocks << lp);
+    S *r = (S *)rho;
+    if (g_pstream_shape.nt) {
+        if (lp == 2) dm_sop_packed_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+    } else {
+        if (lp == 2) dm_sop_packed_kernel<S, 0, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 0, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 0, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+    }
     ORK_LAUNCH_CHECK();
     return 0;
 }

@@ -121,25 +121,29 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
     assert np.array_equal(run(1), run(0))
 
 
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED])
 @pytest.mark.parametrize("fp32", [False, True])
-def test_split_tile_quads_change_no_value(eng, fp32):
-    """The cross-tile superoperator kernel with 2 or 4 workgroups per tile quad (bits 3-4 of the
-    kernel-7 shape flags) forms the same products for every entry as with one: bit-identical."""
+def test_split_tile_quads_change_no_value(eng, kind, fp32):
+    """The streamed superoperator kernels with 2 or 4 workgroups per tile quad (tiled, target above
+    the tile) or per 32 x 32 block of quads (packed) -- bits 3-4 of the kernel-7 / kernel-9 shape
+    flags -- form the same products for every entry as with one: bit-identical."""
     api = eng.api
     n = 9
-    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(45)), MIXED_TILED, 5)
+    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(45)), kind, 5)
     S = wl.superop(wl.kraus_depolarizing(0.05))
+    first = 5 if kind == MIXED_TILED else 0
 
     def run(lp, nt):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
         assert api.orkan_set_launch_shape(7, 8 * lp + 4 * nt, 81920) == 0
-        st = eng.new(MIXED_TILED, n, rho)
+        assert api.orkan_set_launch_shape(9, 8 * lp + 4 * nt + 3, 81920) == 0
+        st = eng.new(kind, n, rho)
         if fp32:
             st32 = api.orkan_state_convert(C.byref(st), 1)
             eng.free(st)
             st = st32
-        for q in range(5, n):
+        for q in range(first, n):
             eng.gate(st, "h", q)
             eng.gate(st, "ry", q, 0.2 + 0.1 * q)
             eng.channel(st, S, q)
