@@ -95,7 +95,9 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9) uses nontemporal loads and stores, bits 3-4
  * (kernel 7, 32 x 32 tiles) give log2 of the workgroups that share one tile quad (0..2);
- * lds_bytes caps workgroups per CU (0: no cap) */
+ * lds_bytes caps workgroups per CU (0: no cap).  The settings are process-wide and unsynchronised:
+ * change them while no other thread is calling into the library.  Returns 0, or hipErrorInvalidValue
+ * for a kernel id or value outside the ranges above. */
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes);
 
 /* ------------------------------------------------------------------------------------------------

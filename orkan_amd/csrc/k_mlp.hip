@@ -614,7 +614,9 @@ extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const
     EpiArgs ep{c32, ldc, bias, pre, (bf16 *)act};
     hipStream_t st = (hipStream_t)stream;
     const bf16 *a = (const bf16 *)A, *b = (const bf16 *)B;
-    if (g_gemm_big == 2 && M % BM2 == 0 && N % BN2 == 0 && K % (4 * BK3) == 0 && K > 0) {
+    // the DMA moves 16 B per lane: rows of both operands must start on 16 B boundaries
+    const bool aligned16 = lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+    if (g_gemm_big == 2 && aligned16 && M % BM2 == 0 && N % BN2 == 0 && K % (4 * BK3) == 0 && K > 0) {
         dim3 grid3(N / BN2, M / BM2);
         switch (epi) {
         case EPI_F32: gemm_nt_dma_kernel<EPI_F32><<<grid3, GT3, 0, st>>>(a, lda, b, ldb, K, ep); break;

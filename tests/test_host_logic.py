@@ -149,3 +149,21 @@ def test_sample_matrix_validation_without_device():
     assert call(C.byref(st), obs, ch, 2, 1, 3 * 16 * 8, S) == -7     # workspace below 4 vectors
     empty = (_ffi.AnsatzChannelT * 1)(_ffi.AnsatzChannelT(0, ops, None))
     assert call(C.byref(st), obs, empty, 1, 0, big, S) == -7
+
+
+def test_launch_shape_hook_ranges_without_device():
+    """The tuning hook only stores host-side settings: valid ids and values are taken, every value
+    outside the documented ranges is refused, and kernel 14 restores the defaults."""
+    import orkan_amd
+    lib = orkan_amd.load(5)
+    ok = [(0, 3, 0), (7, 4 | 16, 49152), (9, 3 | 8, 81920), (10, 0, 81920), (11, 1, 0), (11, 2, 0),
+          (12, 16 | 2 | 4, 53248), (12, 0, 0), (13, 15, 0), (13, 0, 1), (14, 0, 0)]
+    bad = [(-1, 0, 0), (15, 0, 0), (0, -1, 0), (0, 32, 0), (7, 24, 0), (0, 0, -1), (0, 0, 160 * 1024 + 1),
+           (10, 0, -1), (11, 3, 0), (11, -1, 0), (12, 32, 0), (12, 16 | 12, 0), (13, 16, 0), (13, 0, -1)]
+    try:
+        for k, f, l in ok:
+            assert lib.orkan_set_launch_shape(k, f, l) == 0, (k, f, l)
+        for k, f, l in bad:
+            assert lib.orkan_set_launch_shape(k, f, l) != 0, (k, f, l)
+    finally:
+        assert lib.orkan_set_launch_shape(14, 0, 0) == 0
