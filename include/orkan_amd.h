@@ -91,10 +91,12 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * the statevector ones off per family (bit 0 1-qubit gates, bit 1 cx / cz / swap / ccx ..., bit 2
  * exp_diag, bit 3 mean / matel_diag; 0, the default, leaves them on where the tuned shapes use
  * them), 14: every
- * launch shape and tuning choice above back to the built-in defaults (flags and lds_bytes unused));
+ * launch shape and tuning choice above back to the built-in defaults (flags and lds_bytes unused),
+ * 15 / 16: as 6 / 7 for complex64 storage (kernels 6 and 7 then apply to complex128 only));
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
- * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9) uses nontemporal loads and stores, bits 3-4
- * (kernel 7, 32 x 32 tiles) give log2 of the workgroups that share one tile quad (0..2);
+ * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16) uses nontemporal loads and stores,
+ * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9) give log2 of the workgroups that share one tile quad
+ * or packed block (0..2);
  * lds_bytes caps workgroups per CU (0: no cap).  The settings are process-wide and unsynchronised:
  * change them while no other thread is calling into the library.  Returns 0, or hipErrorInvalidValue
  * for a kernel id or value outside the ranges above. */

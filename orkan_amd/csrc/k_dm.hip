@@ -773,8 +773,11 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // where a quad's two columns share 128 B lines (targets below 3 in complex128, below 4 in
 // complex64) they cost 1-2 % and refetch lines (complex64 target 3: 1.1 x the reads), so stay off.
 // The packed kernel's column runs share lines between workgroups: nontemporal there loses 20 %.
-constexpr Shape kStreamShape[2] = {{1, 0, 65536, 1, 0}, {0, 0, 49152, 1, 2}};
-Shape g_stream_shape[2] = {kStreamShape[0], kStreamShape[1]};
+// [in-tile, above the tile] x [complex128, complex64].  Complex64 moves half the bytes per workgroup:
+// tile quads split over 2 workgroups (not 4) keep 16 KB each (6.34 TB/s against 5.24 with 4), and
+// in-tile targets run best in plain order (5.76 against 5.64 TB/s with the remap).
+constexpr Shape kStreamShape[4] = {{1, 0, 65536, 1, 0}, {0, 0, 49152, 1, 2}, {0, 0, 65536, 1, 0}, {0, 0, 49152, 1, 1}};
+Shape g_stream_shape[4] = {kStreamShape[0], kStreamShape[1], kStreamShape[2], kStreamShape[3]};
 int g_stream_on = 1;
 
 template <typename S>
@@ -782,7 +785,7 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     const u64 nt = (u64)1 << (n - L);
     StreamPlan sp;
     sp.L = L;
-    Shape sh = g_stream_shape[q < L ? 0 : 1];
+    Shape sh = g_stream_shape[(q < L ? 0 : 1) + (sizeof(S) == 8 ? 2 : 0)];
     if (((nt * (nt + 1) / 2) << (2 * L)) * sizeof(S) < ork_nt_min_bytes()) sh.nt = 0;
     sp.xcd = sh.xcd;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
@@ -911,13 +914,13 @@ DmPlan tile_plan(int layout, int L, int n) {
 // superoperator with the target qubit inside / above the tile, 8 streamed superoperators on (lds_bytes
 // != 0) or off, 9 streamed packed superoperator; flags bit 0: XCD remap, bit 1: column order of groups,
 // bit 2: nontemporal accesses (phase and streamed kernels), bits 3-4: log2 of the workgroups per tile
-// quad (kernel 7)
+// quad (kernels 7, 16); 15 / 16: as 6 / 7 for complex64 storage
 extern "C" void ork_dm_reset_shapes(void) {
     for (int i = 0; i < 2; ++i) {
         g_group_shape[i] = kGroupShape[i];
         g_phase_shape[i] = kPhaseShape[i];
-        g_stream_shape[i] = kStreamShape[i];
     }
+    for (int i = 0; i < 4; ++i) g_stream_shape[i] = kStreamShape[i];
     g_stream_on = 1;
     g_pstream_shape = kPstreamShape;
 }
@@ -937,6 +940,8 @@ extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
     case 7: g_stream_shape[1] = sh; break;
     case 8: g_stream_on = lds_bytes != 0; break;
     case 9: g_pstream_shape = sh; break;
+    case 15: g_stream_shape[2] = sh; break;
+    case 16: g_stream_shape[3] = sh; break;
     default: return (int)hipErrorInvalidValue;
     }
     return 0;

@@ -92,8 +92,9 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
     def run(nt):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
-        assert api.orkan_set_launch_shape(6, 1 | 4 * nt, 65536) == 0
-        assert api.orkan_set_launch_shape(7, 4 * nt, 81920) == 0
+        for k6, k7 in ((6, 7), (15, 16)):   # complex128 and complex64 shapes
+            assert api.orkan_set_launch_shape(k6, 1 | 4 * nt, 65536) == 0
+            assert api.orkan_set_launch_shape(k7, 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(4, 3 | 4 * nt, 0) == 0
         assert api.orkan_set_launch_shape(5, 4 * nt, 0) == 0
@@ -125,7 +126,7 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_split_tile_quads_change_no_value(eng, kind, fp32):
     """The streamed superoperator kernels with 2 or 4 workgroups per tile quad (tiled, target above
-    the tile) or per 32 x 32 block of quads (packed) -- bits 3-4 of the kernel-7 / kernel-9 shape
+    the tile) or per 32 x 32 block of quads (packed) -- bits 3-4 of the kernel-7 / 16 / 9 shape
     flags -- form the same products for every entry as with one: bit-identical."""
     api = eng.api
     n = 9
@@ -137,6 +138,7 @@ def test_split_tile_quads_change_no_value(eng, kind, fp32):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
         assert api.orkan_set_launch_shape(7, 8 * lp + 4 * nt, 81920) == 0
+        assert api.orkan_set_launch_shape(16, 8 * lp + 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 8 * lp + 4 * nt + 3, 81920) == 0
         st = eng.new(kind, n, rho)
         if fp32:
