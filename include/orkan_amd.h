@@ -96,7 +96,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9) give log2 of the workgroups that share one tile quad
- * or packed block (0..2);
+ * or packed block (0..2), bit 3 of kernels 6, 15 (with bit 2) selects the lane-exchange kernel for
+ * in-tile targets whose quad columns share 128 B lines;
  * lds_bytes caps workgroups per CU (0: no cap).  The settings are process-wide and unsynchronised:
  * change them while no other thread is calling into the library.  Returns 0, or hipErrorInvalidValue
  * for a kernel id or value outside the ranges above. */

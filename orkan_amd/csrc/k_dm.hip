@@ -346,6 +346,58 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
 // PARTS > 1 (32 x 32 tiles only): PARTS consecutive workgroups share one tile quad, each taking
 // 32 / PARTS rows of A, C, D (and of B, or the matching columns of its stored mirror); a diagonal
 // tile quad (TR == TC) stays whole in its first workgroup, since there the mirror is C itself.
+// lane exchange of one complex value as four 32-bit shuffles
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __shfl_xor((int)(unsigned)b, m, 64), hi = __shfl_xor((int)(unsigned)(b >> 32), m, 64);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ cd shfl_xor_cd(cd v, int m) { return cmk(shfl_xor_f64(v.x, m), shfl_xor_f64(v.y, m)); }
+__device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cmk(c ? a.x : b.x, c ? a.y : b.y); }
+
+// In-tile targets whose quad columns c0, c0 | 2^q share 128 B lines: one thread per (row pair, column),
+// lanes walking the columns of a row so that every load and store instruction covers whole lines;
+// the partner column's two entries come from lane ^ 2^q.  Both lanes of a pair form the same four
+// products as the quad kernel above and each stores its own column.
+template <typename S, int U, int NT>
+__global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
+    const int L = sp.L, q = sp.m, T = 1 << L, mb = 1 << q;
+    const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
+    const int per_tile_log = 2 * L - 1;   // T / 2 row pairs x T columns per tile
+    S *pa[U], *pb[U];
+    cd a[U], b[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 Q = (w * U + u) * DM_THREADS + threadIdx.x;
+        ok[u] = Q < sp.nitems;
+        const int within = (int)(Q & (((u64)1 << per_tile_log) - 1));
+        const int r0 = (int)ins0((u64)(within >> L), q), c = within & (T - 1);
+        S *tile = ok[u] ? rho + ((Q >> per_tile_log) << (2 * L)) : rho;
+        pa[u] = tile + ((r0 << L) | c);
+        pb[u] = tile + (((r0 | mb) << L) | c);
+        a[u] = cmk(0.0, 0.0);
+        b[u] = a[u];
+        if (ok[u]) {
+            a[u] = ldx<NT>(pa[u]);
+            b[u] = ldx<NT>(pb[u]);
+        }
+    }
+    // bit q of the column is bit q of the lane (blocks start on multiples of 256 items)
+    const bool lo = !(threadIdx.x & mb);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const cd a2 = shfl_xor_cd(a[u], mb), b2 = shfl_xor_cd(b[u], mb);   // every lane takes part
+        if (!ok[u]) continue;
+        // component-wise selects (a select of the vector type itself goes through memory)
+        const cd v0 = sel(lo, a[u], a2), v1 = sel(lo, a2, a[u]), v2 = sel(lo, b[u], b2), v3 = sel(lo, b2, b[u]);
+        cd o0, o1, o2, o3;
+        sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+        stx<NT>(pa[u], sel(lo, o0, o1));
+        stx<NT>(pb[u], sel(lo, o2, o3));
+    }
+}
+
 template <typename S, int NT, int PARTS>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     __shared__ cd xs[W_MAX * LDS_STRIDE];
@@ -776,7 +828,11 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // [in-tile, above the tile] x [complex128, complex64].  Complex64 moves half the bytes per workgroup:
 // tile quads split over 2 workgroups (not 4) keep 16 KB each (6.34 TB/s against 5.24 with 4), and
 // in-tile targets run best in plain order (5.76 against 5.64 TB/s with the remap).
-constexpr Shape kStreamShape[4] = {{1, 0, 65536, 1, 0}, {0, 0, 49152, 1, 2}, {0, 0, 65536, 1, 0}, {0, 0, 49152, 1, 1}};
+// Bit 3 (lp) of an in-tile shape enables the lane-exchange kernel for targets whose quad columns
+// share 128 B lines; that kernel runs with kLaneShape (in plain order, 48 KB cap for 8 entries per
+// thread: 6.41 against 6.01 TB/s for complex128 targets 0-2, 6.18 against 5.54 for complex64 0-3).
+constexpr Shape kStreamShape[4] = {{1, 0, 65536, 1, 1}, {0, 0, 49152, 1, 2}, {0, 0, 65536, 1, 1}, {0, 0, 49152, 1, 1}};
+constexpr Shape kLaneShape = {0, 0, 49152, 1, 0};
 Shape g_stream_shape[4] = {kStreamShape[0], kStreamShape[1], kStreamShape[2], kStreamShape[3]};
 int g_stream_on = 1;
 
@@ -789,7 +845,18 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
     if (((nt * (nt + 1) / 2) << (2 * L)) * sizeof(S) < ork_nt_min_bytes()) sh.nt = 0;
     sp.xcd = sh.xcd;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
-    if (q < L) {
+    if (q < L && sh.lp && sh.nt && ((u64)sizeof(S) << q) < 128) {
+        // quad columns share lines: the lane-exchange kernel keeps whole lines per instruction, so
+        // nontemporal accesses apply here too; 4 (complex128) or 8 (complex64) entries per thread
+        constexpr int U = sizeof(S) == 8 ? 4 : 2;
+        sp.m = q;
+        sp.xcd = kLaneShape.xcd;
+        sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 1);
+        sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
+        if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
+        dm_sop_intile_lane_kernel<S, U, 1><<<(unsigned)sp.nblocks, DM_THREADS, kLaneShape.lds / (U / 2), st>>>(
+            (S *)rho, sp, op);
+    } else if (q < L) {
         // complex64 storage: two quads per thread and half the LDS cap keep the bytes in flight
         constexpr int U = sizeof(S) == 8 ? 2 : 1;
         sp.m = q;

@@ -93,7 +93,8 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
         for k6, k7 in ((6, 7), (15, 16)):   # complex128 and complex64 shapes
-            assert api.orkan_set_launch_shape(k6, 1 | 4 * nt, 65536) == 0
+            # with nt: the lane-exchange kernel (bit 3) for in-tile targets whose quad columns share lines
+            assert api.orkan_set_launch_shape(k6, 1 | 12 * nt, 65536) == 0
             assert api.orkan_set_launch_shape(k7, 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(4, 3 | 4 * nt, 0) == 0
