@@ -16,7 +16,9 @@ __device__ __forceinline__ cd cmk(double re, double im) { cd v; v.x = re; v.y = 
 __device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
-__device__ __forceinline__ cd cmul(cd a, cd b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// products are fused explicitly (fma of one product into the other), so every kernel that forms a
+// given product rounds it the same way whatever the surrounding code lets the compiler contract
+__device__ __forceinline__ cd cmul(cd a, cd b) { return cmk(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)); }
 __device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
 __device__ __forceinline__ cd cneg(cd a) { return cmk(-a.x, -a.y); }
 // multiply by i^k, k in 0..3 (exact)
@@ -71,6 +73,16 @@ template <> struct Store<float2> {
 };
 template <typename S> __device__ __forceinline__ cd ldc(const S *p) { return Store<S>::ld(p); }
 template <typename S> __device__ __forceinline__ void stc(S *p, cd v) { Store<S>::st(p, v); }
+// lane exchange (within a wavefront) of one complex value as four 32-bit shuffles
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __shfl_xor((int)(unsigned)b, m, 64), hi = __shfl_xor((int)(unsigned)(b >> 32), m, 64);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ cd shfl_xor_cd(cd v, int m) { return cmk(shfl_xor_f64(v.x, m), shfl_xor_f64(v.y, m)); }
+// component-wise select (a select of the vector type itself goes through memory)
+__device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cmk(c ? a.x : b.x, c ? a.y : b.y); }
+
 // NT selects the nontemporal forms
 template <int NT, typename S> __device__ __forceinline__ cd ldx(const S *p) {
     if constexpr (NT) return Store<S>::ldn(p);

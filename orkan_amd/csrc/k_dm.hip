@@ -346,14 +346,6 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
 // PARTS > 1 (32 x 32 tiles only): PARTS consecutive workgroups share one tile quad, each taking
 // 32 / PARTS rows of A, C, D (and of B, or the matching columns of its stored mirror); a diagonal
 // tile quad (TR == TC) stays whole in its first workgroup, since there the mirror is C itself.
-// lane exchange of one complex value as four 32-bit shuffles
-__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const int lo = __shfl_xor((int)(unsigned)b, m, 64), hi = __shfl_xor((int)(unsigned)(b >> 32), m, 64);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-__device__ __forceinline__ cd shfl_xor_cd(cd v, int m) { return cmk(shfl_xor_f64(v.x, m), shfl_xor_f64(v.y, m)); }
-__device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cmk(c ? a.x : b.x, c ? a.y : b.y); }
 
 // In-tile targets whose quad columns c0, c0 | 2^q share 128 B lines: one thread per (row pair, column),
 // lanes walking the columns of a row so that every load and store instruction covers whole lines;

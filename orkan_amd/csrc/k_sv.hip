@@ -41,18 +41,18 @@ __device__ __forceinline__ void pair_op(cd &a, cd &b, const Coef &k) {
         a = na; b = nb;
     } else if constexpr (OP == ORK_SV_RX) {
         const double c = k.v[0], s = k.v[1];
-        const cd na = cmk(c * a.x + s * b.y, c * a.y - s * b.x);
-        const cd nb = cmk(s * a.y + c * b.x, -s * a.x + c * b.y);
+        const cd na = cmk(fma(c, a.x, s * b.y), fma(c, a.y, -(s * b.x)));
+        const cd nb = cmk(fma(s, a.y, c * b.x), fma(-s, a.x, c * b.y));
         a = na; b = nb;
     } else if constexpr (OP == ORK_SV_RY) {
         const double c = k.v[0], s = k.v[1];
-        const cd na = cmk(c * a.x - s * b.x, c * a.y - s * b.y);
-        const cd nb = cmk(s * a.x + c * b.x, s * a.y + c * b.y);
+        const cd na = cmk(fma(c, a.x, -(s * b.x)), fma(c, a.y, -(s * b.y)));
+        const cd nb = cmk(fma(s, a.x, c * b.x), fma(s, a.y, c * b.y));
         a = na; b = nb;
     } else if constexpr (OP == ORK_SV_RZ) {
         const double c = k.v[0], s = k.v[1];
-        const cd na = cmk(c * a.x + s * a.y, c * a.y - s * a.x);
-        const cd nb = cmk(c * b.x - s * b.y, c * b.y + s * b.x);
+        const cd na = cmk(fma(c, a.x, s * a.y), fma(c, a.y, -(s * a.x)));
+        const cd nb = cmk(fma(c, b.x, -(s * b.y)), fma(c, b.y, s * b.x));
         a = na; b = nb;
     }
 }
@@ -62,7 +62,7 @@ template <int OP>
 __device__ __forceinline__ cd one_op(cd b, const Coef &k) {
     if constexpr (OP == ORK_SV_P) {
         const double c = k.v[0], s = k.v[1];
-        return cmk(c * b.x - s * b.y, s * b.x + c * b.y);
+        return cmk(fma(c, b.x, -(s * b.y)), fma(s, b.x, c * b.y));
     } else if constexpr (OP == ORK_SV_Z) {
         return cneg(b);
     } else if constexpr (OP == ORK_SV_S) {
@@ -90,9 +90,10 @@ struct SvShape {
 
 // best shape per target qubit of a 2^28-amplitude state (sweep of xcd x u x LDS cap x nontemporal).
 // Nontemporal accesses gain 6-10 % from target 3 up; below that a pair shares its 128 B line with
-// the next pair, which the cache then has to supply twice.
+// the next pair, which the cache then has to supply twice, so targets 0-2 take them through the
+// lane-exchange kernel (6.68 against 6.36 TB/s).
 static const int kPairShapeHost[28][4] = {
-    {1, 1, 53248, 0}, {1, 1, 53248, 0}, {1, 1, 53248, 0}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1},
+    {1, 1, 32768, 1}, {1, 1, 32768, 1}, {1, 1, 32768, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1},
     {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1},
     {1, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1}, {0, 1, 53248, 1},
     {1, 1, 40960, 1}, {1, 1, 40960, 1}, {1, 1, 53248, 1}, {1, 1, 53248, 1}, {1, 1, 40960, 1}, {0, 1, 53248, 1},
@@ -100,9 +101,10 @@ static const int kPairShapeHost[28][4] = {
 
 // complex64 storage (pairs per thread and LDS cap as launched): twice the pairs and half the cap of the
 // complex128 shape, except where a sweep of complex64 shapes on 2^28 amplitudes found better
-// (targets 0-3 and several above 17: 2 pairs per thread in plain order, 77.7 -> 79.1 % over the 28)
+// (several targets above 17: 2 pairs per thread in plain order, 77.7 -> 79.1 % over the 28; targets
+// 0-3 through the lane-exchange kernel, uncapped: 6.35 against 5.5-5.85 TB/s)
 static const int kPairShapeF32[28][4] = {
-    {0, 2, 40960, 0}, {0, 2, 40960, 0}, {0, 2, 32768, 0}, {0, 2, 32768, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1},
+    {0, 2, 0, 1}, {0, 2, 0, 1}, {0, 2, 0, 1}, {0, 2, 0, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1},
     {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1}, {1, 2, 26624, 1},
     {1, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1}, {0, 2, 26624, 1},
     {0, 2, 20480, 1}, {0, 2, 20480, 1}, {0, 2, 20480, 1}, {0, 4, 20480, 1}, {1, 2, 20480, 1}, {1, 2, 20480, 1},
@@ -172,6 +174,30 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
             stx<NT>(psi + i0[u], a[u]);
             stx<NT>(psi + i0[u] + stride, b[u]);
         }
+    }
+}
+
+// Targets whose pair shares a 128 B line: one amplitude per lane, lanes contiguous so that every
+// instruction covers whole lines (nontemporal); the partner amplitude comes from lane ^ 2^q.  Both
+// lanes of a pair form the same update and each stores its own amplitude.
+template <typename S, int OP, int U, int XCD>
+__global__ __launch_bounds__(SV_THREADS) void sv_pair_lane_kernel(S *__restrict__ psi, u64 dim, int q, Coef k) {
+    const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
+    const int mb = 1 << q;
+    const bool lo = !(threadIdx.x & mb);   // blocks start on multiples of 256: bit q of x is bit q of the lane
+    cd v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 x = base + (u64)u * SV_THREADS;
+        v[u] = x < dim ? ldx<1>(psi + x) : cmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 x = base + (u64)u * SV_THREADS;
+        const cd w = shfl_xor_cd(v[u], mb);   // every lane takes part
+        cd a = sel(lo, v[u], w), b = sel(lo, w, v[u]);
+        pair_op<OP>(a, b, k);
+        if (x < dim) stx<1>(psi + x, sel(lo, a, b));
     }
 }
 
@@ -361,14 +387,35 @@ void launch_nt(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k, int 
     else launch_shape<S, OP, U, XCD, 0>(st, psi, npairs, q, k, lds);
 }
 
+template <typename S, int OP, int XCD>
+void launch_lane(hipStream_t st, void *psi, u64 dim, int q, const Coef &k, int u, int lds) {
+    const u64 per = (u64)SV_THREADS * u;
+    const unsigned grid = (unsigned)((dim + per - 1) / per);
+    if (u >= 8) sv_pair_lane_kernel<S, OP, 8, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, dim, q, k);
+    else if (u >= 4) sv_pair_lane_kernel<S, OP, 4, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, dim, q, k);
+    else sv_pair_lane_kernel<S, OP, 2, XCD><<<grid, SV_THREADS, lds, st>>>((S *)psi, dim, q, k);
+}
+
 template <typename S, int OP>
 void launch_pair(hipStream_t st, void *psi, u64 npairs, int q, const Coef &k) {
     SvShape sh = pair_shape(q, sizeof(S) == 8);
     // the remap needs a grid of at least 8 blocks; small states use the plain order
     if (npairs < (u64)SV_THREADS * sh.u * 8) sh.xcd = 0;
-    // nontemporal only on large states, and only where a pair's two amplitudes lie in different
-    // 128 B lines (target 3 up in complex128, 4 up in complex64)
-    if (2 * npairs * sizeof(S) < g_nt_min_bytes || ((u64)sizeof(S) << q) < 128) sh.nt = 0;
+    // nontemporal only on large states
+    if (2 * npairs * sizeof(S) < g_nt_min_bytes) sh.nt = 0;
+    constexpr bool one = OP == ORK_SV_P || OP == ORK_SV_Z || OP == ORK_SV_S || OP == ORK_SV_SDG || OP == ORK_SV_T ||
+                         OP == ORK_SV_TDG;
+    if (((u64)sizeof(S) << q) < 128) {
+        // a pair's two amplitudes share a 128 B line (target below 3 in complex128, below 4 in
+        // complex64): nontemporal accesses only through the lane-exchange kernel, with the same bytes
+        // in flight per thread (2 u amplitudes)
+        if (sh.nt && !one) {
+            if (sh.xcd) launch_lane<S, OP, 1>(st, psi, 2 * npairs, q, k, 2 * sh.u, sh.lds);
+            else launch_lane<S, OP, 0>(st, psi, 2 * npairs, q, k, 2 * sh.u, sh.lds);
+            return;
+        }
+        sh.nt = 0;
+    }
     if (sh.u == 4) {
         if (sh.xcd) launch_nt<S, OP, 4, 1>(st, psi, npairs, q, k, sh.lds, sh.nt);
         else launch_nt<S, OP, 4, 0>(st, psi, npairs, q, k, sh.lds, sh.nt);

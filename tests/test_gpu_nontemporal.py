@@ -29,9 +29,11 @@ def eng():
     assert e.api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
 
 
-def _sv_run(eng, n, fp32, off, psi, obs_dev):
+def _sv_run(eng, n, fp32, off, psi, obs_dev, shape=None):
     api = eng.api
     assert api.orkan_set_launch_shape(NT_OFF_HOOK, off, 0) == 0
+    if shape is not None:
+        assert api.orkan_set_launch_shape(12, *shape) == 0
     st = eng.new(PURE, n, psi)
     if fp32:
         st32 = api.orkan_state_convert(C.byref(st), 1)
@@ -74,10 +76,12 @@ def test_statevector_streaming_accesses_change_no_value(eng, fp32):
     try:
         a, ma, ea = _sv_run(eng, n, fp32, 0, psi, obs_dev)
         b, mb, eb = _sv_run(eng, n, fp32, 15, psi, obs_dev)
+        # every target nontemporal: the lane-exchange kernel for targets whose pair shares a line
+        c, mc, ec = _sv_run(eng, n, fp32, 0, psi, obs_dev, shape=(16 | 2 | 1, 53248))
     finally:
         eng.api.orkan_dev_free(obs_dev)
-    assert np.array_equal(a, b)
-    assert ma == mb and ea == eb
+    assert np.array_equal(a, b) and np.array_equal(c, b)
+    assert ma == mb == mc and ea == eb == ec
 
 
 @pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED])
