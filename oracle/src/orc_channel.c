@@ -13,9 +13,9 @@
  * channel_2q has no counterpart in the reference (channel_1q rejects n_qubits != 1,
  * src/channel/channel.c:71; kraus_to_superop already builds n-qubit superoperators,
  * src/channel/channel.c:24-47): every 4x4 block of the qubit pair, read and written through
- * the Hermitian state_get / state_set, gets out = S vec(block).  Parity unpinned by the
- * reference; pinned by tests/test_oracle_math.py against sum_a K_a rho K_a^dagger on dense
- * matrices.
+ * the Hermitian state_get / state_set, gets out = S vec(block).  With no reference function to
+ * compare against, it is pinned by tests/test_oracle_math.py against sum_a K_a rho K_a^dagger
+ * on dense matrices (every ordered pair, both layouts and tile sizes).
  */
 #include "channel.h"
 #include "orc_internal.h"
