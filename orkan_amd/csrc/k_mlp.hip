@@ -378,7 +378,17 @@ __global__ __launch_bounds__(256) void head_forward_kernel(const bf16 *__restric
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= M) return;
     float s = 0.0f;
-    for (int k = lane; k < K; k += 64) s += (float)h[(long)row * K + k] * w[k];
+    if (K % 8 == 0 && ((uintptr_t)h & 15) == 0 && ((uintptr_t)w & 15) == 0) {
+        // 16 B per lane: one load instruction covers 512 activations of the row
+        for (int k = lane * 8; k < K; k += 512) {
+            const bf16x8 hv = *(const bf16x8 *)(h + (long)row * K + k);
+            const float4 w0 = *(const float4 *)(w + k), w1 = *(const float4 *)(w + k + 4);
+            s += (float)hv[0] * w0.x + (float)hv[1] * w0.y + (float)hv[2] * w0.z + (float)hv[3] * w0.w +
+                 (float)hv[4] * w1.x + (float)hv[5] * w1.y + (float)hv[6] * w1.z + (float)hv[7] * w1.w;
+        }
+    } else {
+        for (int k = lane; k < K; k += 64) s += (float)h[(long)row * K + k] * w[k];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0 && (target || row < valid)) {   // forward only: no writes past the valid rows
