@@ -493,34 +493,33 @@ __global__ void first_layer_rows_kernel(const float *__restrict__ angles, int na
                                         const float *__restrict__ u, const bf16 *__restrict__ wa,
                                         const float *__restrict__ bias, int N, bf16 *__restrict__ act) {
     const int na = NA > 0 ? NA : na_rt;
-    const unsigned nq = (unsigned)(N >> 2);
+    // 8 outputs per thread: 16 B weight loads and one 16 B store
+    const unsigned nq = (unsigned)(N >> 3);
     const unsigned total = (unsigned)Mp * nq;
     for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
         const unsigned r = i / nq;
-        const int j = (int)(i - r * nq) * 4;
-        bf16x4 out;
+        const int j = (int)(i - r * nq) * 8;
+        bf16x8 out;
         if ((int)r < rows) {
-            const float4 u4 = *(const float4 *)(u + j);
-            float z0 = u4.x, z1 = u4.y, z2 = u4.z, z3 = u4.w;
+            const float4 u0 = *(const float4 *)(u + j), u1 = *(const float4 *)(u + j + 4);
+            float z[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
             const float *a = angles + (r0 + (long)r) * na;
 #pragma unroll 8
             for (int k = 0; k < na; ++k) {
                 const float ak = (float)(bf16)a[k];
-                const bf16x4 w = *(const bf16x4 *)(wa + (long)k * N + j);
-                z0 += ak * (float)w[0];
-                z1 += ak * (float)w[1];
-                z2 += ak * (float)w[2];
-                z3 += ak * (float)w[3];
+                const bf16x8 w = *(const bf16x8 *)(wa + (long)k * N + j);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) z[t] += ak * (float)w[t];
             }
-            const float4 b4 = *(const float4 *)(bias + j);
-            out[0] = (bf16)gelu(z0 + b4.x);
-            out[1] = (bf16)gelu(z1 + b4.y);
-            out[2] = (bf16)gelu(z2 + b4.z);
-            out[3] = (bf16)gelu(z3 + b4.w);
+            const float4 b0 = *(const float4 *)(bias + j), b1 = *(const float4 *)(bias + j + 4);
+            const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) out[t] = (bf16)gelu(z[t] + bb[t]);
         } else {
-            out[0] = out[1] = out[2] = out[3] = (bf16)0.0f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) out[t] = (bf16)0.0f;
         }
-        *(bf16x4 *)(act + (long)r * N + j) = out;
+        *(bf16x8 *)(act + (long)r * N + j) = out;
     }
 }
 
@@ -714,8 +713,8 @@ extern "C" int ork_mlp_first_layer_graph(void *stream, const float *gf, int ng, 
 
 extern "C" int ork_mlp_first_layer_rows(void *stream, const float *angles, int na, long r0, int rows, int Mp,
                                         const float *u, const void *wa, const float *bias, int N, void *act) {
-    if (N % 4 || (long)Mp * (N / 4) >= (1L << 32)) return (int)hipErrorInvalidValue;
-    const unsigned grid = g1((long)Mp * (N / 4));
+    if (N % 8 || (long)Mp * (N / 8) >= (1L << 32)) return (int)hipErrorInvalidValue;
+    const unsigned grid = g1((long)Mp * (N / 8));
     if (na == 8)
         first_layer_rows_kernel<8><<<grid, 256, 0, (hipStream_t)stream>>>(angles, na, r0, rows, Mp, u,
                                                                          (const bf16 *)wa, bias, N, (bf16 *)act);
