@@ -31,7 +31,8 @@ enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_BWD = 2, EPI_F32_ACC = 3 };
 // dominated the GEMM epilogue.
 __device__ __forceinline__ float erf_as(float x) {
     const float ax = fabsf(x);
-    const float t = 1.0f / fmaf(0.3275911f, ax, 1.0f);
+    // hardware reciprocal (1 ulp): the approximation's own error is 1.5e-7
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
     const float poly =
         t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
     return copysignf(1.0f - poly * __expf(-ax * ax), x);
