@@ -84,12 +84,23 @@ def test_statevector_streaming_accesses_change_no_value(eng, fp32):
     assert ma == mb == mc and ea == eb == ec
 
 
+@pytest.fixture(scope="module")
+def eng3():
+    e = engines.DeviceEngine(orkan_amd.lib_path(3), 3)
+    yield e
+    assert e.api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+
+
+@pytest.mark.parametrize("lt", [5, 3])
 @pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED])
 @pytest.mark.parametrize("fp32", [False, True])
-def test_density_matrix_streaming_accesses_change_no_value(eng, kind, fp32):
+def test_density_matrix_streaming_accesses_change_no_value(eng, eng3, lt, kind, fp32):
+    """Both tile sizes: 32 x 32 and 8 x 8 (where the lane-exchange kernel pairs lanes within rows of
+    8 columns)."""
+    eng = eng if lt == 5 else eng3
     api = eng.api
     n = 10
-    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(43)), kind, 5)
+    rho = to_layout(wl.low_rank_rho(n, 16, wl.rng(43)), kind, lt)
     S = wl.superop(wl.kraus_amp_damp(0.05))
     diag = wl.rng(44).uniform(-1, 1, 1 << n)
 
