@@ -50,6 +50,7 @@ struct EpiArgs {
     const float *bias;   // EPI_BIAS_GELU: bias[N]
     float *pre;          // EPI_BIAS_GELU: pre-activation out (fp32, ldc); EPI_GELU_BWD: pre-activation in
     bf16 *act;           // EPI_BIAS_GELU: gelu(pre) out (bf16, ldc); EPI_GELU_BWD: dPre out (bf16, ldc)
+    long zstride;        // split K (gemm_nt_kernel, EPI_F32): c32 offset between the gridDim.z slices
 };
 
 template <int EPI>
@@ -58,6 +59,11 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const bf16 *__restrict__ A,
     __shared__ __attribute__((aligned(16))) bf16 As[2][BM * KP];
     __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN * KP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (gridDim.z > 1) {   // split K: slice z covers columns [z K, z K + K) of A and B
+        A += (long)blockIdx.z * K;
+        B += (long)blockIdx.z * K;
+        ep.c32 += (long)blockIdx.z * ep.zstride;
+    }
     const int wm = wave >> 1, wn = wave & 1;
     const long m0 = (long)blockIdx.y * BM, n0 = (long)blockIdx.x * BN;
     // global -> register staging: each thread moves 16 bf16 of A and 16 of B per K step
@@ -658,6 +664,34 @@ extern "C" int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const
     case EPI_F32_ACC: gemm_nt_kernel<EPI_F32_ACC><<<grid, GT, 0, st>>>(a, lda, b, ldb, K, ep); break;
     default: return (int)hipErrorInvalidValue;
     }
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+// c[r][col] = sum over the slices, in slice order (deterministic)
+__global__ void splitk_sum_kernel(const float *__restrict__ work, int slices, long mn, int N, float *__restrict__ c,
+                                  int ldc) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < mn; i += (long)gridDim.x * 256) {
+        float s = 0.0f;
+        for (int z = 0; z < slices; ++z) s += work[z * mn + i];
+        c[(i / N) * ldc + i % N] = s;
+    }
+}
+
+// C = A B^T with K split into `slices` equal slices on the 128 x 128 tile (fp32 partial products in
+// work[slices][M][N], then summed in slice order): the batch-length weight-gradient products, whose
+// M x N has only a few tiles
+extern "C" int ork_mlp_gemm_splitk(void *stream, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
+                                   int slices, float *work, float *c32, int ldc) {
+    if (slices < 1 || slices > 65535 || K % slices || (K / slices) % BK || K <= 0 || M % BM || N % BN || M <= 0 ||
+        N <= 0)
+        return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    EpiArgs ep{work, N, nullptr, nullptr, nullptr, (long)M * N};
+    dim3 grid(N / BN, M / BM, slices);
+    gemm_nt_kernel<EPI_F32><<<grid, GT, 0, st>>>((const bf16 *)A, lda, (const bf16 *)B, ldb, K / slices, ep);
+    ORK_LAUNCH_CHECK();
+    splitk_sum_kernel<<<g1((long)M * N), 256, 0, st>>>(work, slices, (long)M * N, N, c32, ldc);
     ORK_LAUNCH_CHECK();
     return 0;
 }

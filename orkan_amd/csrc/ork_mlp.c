@@ -16,6 +16,8 @@
 
 int ork_mlp_gemm(void *stream, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
                  float *c32, int ldc, const float *bias, float *pre, void *act);
+int ork_mlp_gemm_splitk(void *stream, const void *A, int lda, const void *B, int ldb, int M, int N, int K, int slices,
+                        float *work, float *c32, int ldc);
 int ork_mlp_transpose(void *stream, const void *in, void *out, int R, int Cc);
 int ork_mlp_to_bf16(void *stream, const float *in, void *out, long n);
 int ork_mlp_head_forward(void *stream, const void *h, const float *w, float b, int M, int K, float *y,
@@ -30,6 +32,7 @@ int ork_mlp_adam(void *stream, float *w, const float *g, float *m, float *v, lon
                  float b2, float eps, float c1, float c2, void *wb, void *wtb, int nrows);
 
 #define MLP_MAX_LAYERS 4
+#define SPLITK_MAX 32   /* K slices of the weight-gradient products */
 #define TILE 128
 
 typedef struct {
@@ -54,6 +57,7 @@ struct orkan_mlp {
     void *tA, *tB;              /* bf16 transposes [hidden][M] */
     float *y, *dy, *sq, *tgt, *scal;
     float *colpart;             /* 64 x hidden partial column sums */
+    float *splitk;              /* SPLITK_MAX x max(in_dim, hidden) x hidden split-K partial products */
     float *gf_dev, *u1;         /* screening: graph features [in_dim], their layer-1 share [hidden] */
     long step;
 };
@@ -109,6 +113,7 @@ orkan_mlp_t *orkan_mlp_create(int in_dim, int hidden, int nhidden, int max_batch
     q->sq = dmalloc(M * 4, &err); q->tgt = dmalloc(M * 4, &err);
     q->scal = dmalloc(64, &err);
     q->colpart = dmalloc((size_t)64 * H * 4, &err);
+    q->splitk = dmalloc((size_t)SPLITK_MAX * (size_t)(in_dim > H ? in_dim : H) * H * 4, &err);
     q->gf_dev = dmalloc((size_t)in_dim * 4, &err);
     q->u1 = dmalloc(H * 4, &err);
     if (err) {
@@ -140,7 +145,7 @@ void orkan_mlp_destroy(orkan_mlp_t *q) {
             if (p[i]) (void)hipFree(p[i]);
     }
     void *p[] = {q->w_out, q->g_out, q->m_out, q->v_out, q->gb_out_d, q->x32, q->x16, q->dpre, q->dpre2,
-                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal, q->colpart, q->gf_dev, q->u1};
+                 q->tA, q->tB, q->y, q->dy, q->sq, q->tgt, q->scal, q->colpart, q->splitk, q->gf_dev, q->u1};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i)
         if (p[i]) (void)hipFree(p[i]);
     free(q);
@@ -280,7 +285,15 @@ static int backward(orkan_mlp_t *q, int rows, const float *target, float *loss) 
         /* dW_l = X^T dP: transposes to [K][M] and [N][M], then C = A B^T over the batch */
         e = ork_mlp_transpose(st, xin, q->tA, Mp, L->K);
         if (!e) e = ork_mlp_transpose(st, dcur, q->tB, Mp, L->N);
-        if (!e) e = ork_mlp_gemm(st, 0, q->tA, Mp, q->tB, Mp, L->K, L->N, Mp, L->g, L->N, NULL, NULL, NULL);
+        if (!e) {
+            /* K = the batch (thousands) but only (K_l / 128) x (N / 128) output tiles: split the batch
+             * into up to SPLITK_MAX slices of >= 128 rows until ~512 workgroups run */
+            const int tiles = (L->K / 128) * (L->N / 128);
+            int slices = 1;
+            while (slices < SPLITK_MAX && tiles * slices < 512 && Mp % (slices * 2 * 128) == 0) slices *= 2;
+            e = slices > 1 ? ork_mlp_gemm_splitk(st, q->tA, Mp, q->tB, Mp, L->K, L->N, Mp, slices, q->splitk, L->g, L->N)
+                           : ork_mlp_gemm(st, 0, q->tA, Mp, q->tB, Mp, L->K, L->N, Mp, L->g, L->N, NULL, NULL, NULL);
+        }
         if (!e) e = ork_mlp_colsum(st, dcur, NULL, Mp, L->N, L->gb, q->colpart);
         if (!e && l > 0) {
             /* dP_{l-1} = (dP_l W_l^T) * gelu'(pre_{l-1}) */
