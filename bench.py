@@ -176,6 +176,8 @@ def run_sv28(args, lib):
     per_gate["matel_diag"] = round((2 * elem + 8) * (1 << n) / (matel_ms * 1e-3) / 1e9, 1) if matel_ms > 0 else None
     per_gate["exp_diag"] = round((2 * elem + 8) * (1 << n) / (expd_ms * 1e-3) / 1e9, 1) if expd_ms > 0 else None
 
+    fused = fused_layer(lib, ps, n, elem, layers[args.warmup] if steps > 0 else layers[0], rot_bytes, rot_avg)
+
     value = total_bytes / (total_ms * 1e-3) / 1e9
     lib.orkan_dev_free(obs_dev)
     lib.state_free(ps)
@@ -204,6 +206,7 @@ def run_sv28(args, lib):
         "per_gate_GBs": per_gate,
         "pct_peak_per_gate": {k: (round(v / PEAK_HBM_GBS * 100, 1) if v else None) for k, v in per_gate.items()},
         "mean_value": mval,
+        "fused_layer": fused,
         "wall_s": round(wall, 3),
         "step_ms_minmax": [round(min(step_ms), 3), round(max(step_ms), 3)],
     }
@@ -212,6 +215,44 @@ def run_sv28(args, lib):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _rot_matrix(name, th):
+    c, s = np.cos(th / 2), np.sin(th / 2)
+    if name == "rx":
+        return np.array([[c, -1j * s], [-1j * s, c]])
+    if name == "ry":
+        return np.array([[c, -s], [s, c]])
+    return np.array([[np.exp(-0.5j * th), 0], [0, np.exp(0.5j * th)]])
+
+
+def fused_layer(lib, ps, n, elem, layer, rot_bytes, rot_avg_ms):
+    """The layer's rx, ry, rz on every qubit as one orkan_layer_1q call (U = RZ RY RX per qubit),
+    timed over three calls; reported apart from the per-gate figures, which time one gate per call."""
+    ang = {}
+    for op in layer:
+        if op[0] != "cx":
+            ang.setdefault(op[1], {})[op[0]] = op[2]
+    mats = []
+    for q in range(n):
+        a = ang.get(q, {})
+        mats.append(_rot_matrix("rz", a.get("rz", 0.0)) @ _rot_matrix("ry", a.get("ry", 0.0))
+                    @ _rot_matrix("rx", a.get("rx", 0.0)))
+    m = np.ascontiguousarray(np.concatenate([M.T.reshape(4) for M in mats]), np.complex128)
+    ev = Events(lib)
+    lib.orkan_sync()
+    f0 = ev.rec()
+    for _ in range(3):
+        assert lib.orkan_layer_1q(ps, m.ctypes.data) == 0
+    f1 = ev.rec()
+    lib.orkan_sync()
+    ms = ev.ms(f0, f1) / 3
+    rots = sum(1 for op in layer if op[0] != "cx")
+    sweeps = 1 + max(0, -(-(n - 12) // 8))
+    return {"rotations": rots, "ms": round(ms, 4), "sweeps": sweeps,
+            "hbm_GBs": round(sweeps * 2 * elem * (1 << n) / (ms * 1e-3) / 1e9, 1),
+            "speedup_vs_gate_calls": round(rots * rot_avg_ms / ms, 2),
+            "note": "fused rotations of one layer, one call: HBM GB/s on the bytes the sweeps move; not a per-gate figure"}
 
 
 def sv_fp32_block(lib, layers, n, nlayers=2):

@@ -47,6 +47,17 @@ qs_error_t orkan_state_save(const state_t *state, const char *path);
 qs_error_t orkan_state_load(state_t *state, const char *path);
 
 /* ------------------------------------------------------------------------------------------------
+ * Fused layer of single-qubit gates (no counterpart in the reference, which applies one gate per
+ * call; the reference's survey of next steps names gate fusion, reported apart from the per-gate
+ * figures).  Applies mats[4 q .. 4 q + 3], the 2x2 matrix of qubit q in column-major order (the
+ * single_from_mat convention), to every qubit q of a PURE state: the same state as calling the gates
+ * one by one (they act on distinct qubits and commute), to rounding.  The statevector moves through
+ * HBM in 1 + ceil((n - 12) / 8) LDS-tiled sweeps instead of n passes.  QS_ERR_NULL, QS_ERR_TYPE (not
+ * PURE).
+ * ---------------------------------------------------------------------------------------------- */
+qs_error_t orkan_layer_1q(state_t *state, const cplx_t *mats);
+
+/* ------------------------------------------------------------------------------------------------
  * Bulk transfer of the storage array (state_len elements in the state's own
  * precision: cplx_t, or float pairs for ORKAN_FP32).  Complements state_get /
  * state_set (reference include/state.h:128,141), which move one element.

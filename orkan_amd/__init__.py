@@ -24,6 +24,7 @@ EXTRA = {
     "orkan_state_download": (C.c_int, [_ffi.PS, C.c_void_p]),
     "orkan_state_convert": (_ffi.StateT, [_ffi.PS, C.c_int]),
     "orkan_state_relayout": (C.c_int, [_ffi.PS, _ffi.PS, C.c_int]),
+    "orkan_layer_1q": (C.c_int, [_ffi.PS, C.c_void_p]),
     "orkan_state_save": (C.c_int, [_ffi.PS, C.c_char_p]),
     "orkan_state_load": (C.c_int, [_ffi.PS, C.c_char_p]),
     "orkan_dev_alloc": (C.c_void_p, [C.c_uint64]),

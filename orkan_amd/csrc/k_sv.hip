@@ -452,10 +452,116 @@ int sv_1q(hipStream_t st, void *psi, int n, int q, int op, const Coef &k) {
     return 0;
 }
 
+
+// ---- fused layer of single-qubit gates ----------------------------------------------------
+// One sweep loads tiles of 2^tb amplitudes into LDS, applies the 2x2 matrices of the qubits the
+// tile spans and writes the tile back.  Tile element e maps to the amplitude
+//   x = low | (tmid << r) | ((e >> r) << lo) | (thigh << (lo + g)),   low = e & (2^r - 1),
+// so a tile holds 2^r consecutive amplitudes (whole lines) times the 2^g values of qubits
+// lo..lo+g-1; the first sweep (r = tb, g = 0) takes qubits 0..tb-1, each later one the next g
+// qubits, so every qubit's matrix is applied exactly once.
+constexpr int LAYER_LT = 12, LAYER_R = 4;
+struct LayerArgs {
+    double m[LAYER_LT][8];   // per tile bit: m00, m01, m10, m11 (re, im), row-major
+    int tb, r, lo, g;        // tile bits; low run bits; global bit of tile bit r; gate bits above the run
+    int b0, b1;              // tile bits [b0, b1) carry a matrix
+};
+
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }   // one spare slot per 8 amplitudes
+
+template <typename S, int NT>
+__global__ __launch_bounds__(SV_THREADS) void sv_layer_kernel(S *__restrict__ psi, LayerArgs a) {
+    __shared__ cd tile[(1 << LAYER_LT) + ((1 << LAYER_LT) >> 3)];
+    const int T = 1 << a.tb;
+    const u64 t = blockIdx.x;
+    const int midb = a.lo - a.r;
+    const u64 tmid = t & (((u64)1 << midb) - 1), thigh = t >> midb;
+    const u64 base = (tmid << a.r) | (thigh << (a.lo + a.g));
+#define LIDX(e) (base | (u64)((e) & ((1 << a.r) - 1)) | ((u64)((e) >> a.r) << a.lo))
+    for (int e = threadIdx.x; e < T; e += SV_THREADS) tile[lpad(e)] = ldx<NT>(psi + LIDX(e));
+    __syncthreads();
+    // up to three tile bits per pass: each thread holds the 8 amplitudes of one 3-bit group
+    for (int b = a.b0; b < a.b1;) {
+        const int rb = (a.b1 - b) < 3 ? (a.b1 - b) : 3;
+        const int R = 1 << rb, ngroups = T >> rb, lowmask = (1 << b) - 1;
+        for (int gi = threadIdx.x; gi < ngroups; gi += SV_THREADS) {
+            const int g0 = ((gi & ~lowmask) << rb) | (gi & lowmask);
+            cd v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < R) v[j] = tile[lpad(g0 | (j << b))];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= rb) break;
+                const double *m = a.m[b + k];
+                const cd m00 = cmk(m[0], m[1]), m01 = cmk(m[2], m[3]), m10 = cmk(m[4], m[5]), m11 = cmk(m[6], m[7]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < R && !((j >> k) & 1)) {
+                        cd x0 = cmul(m00, v[j]), x1 = cmul(m10, v[j]);
+                        cfma(x0, m01, v[j | (1 << k)]);
+                        cfma(x1, m11, v[j | (1 << k)]);
+                        v[j] = x0;
+                        v[j | (1 << k)] = x1;
+                    }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < R) tile[lpad(g0 | (j << b))] = v[j];
+        }
+        __syncthreads();
+        b += rb;
+    }
+    for (int e = threadIdx.x; e < T; e += SV_THREADS) stx<NT>(psi + LIDX(e), tile[lpad(e)]);
+#undef LIDX
+}
+
+template <typename S>
+int sv_layer(hipStream_t st, void *psi, int n, const double *mats) {
+    const bool nt = !(g_nt_off & 1) && ((u64)sizeof(S) << n) >= g_nt_min_bytes;
+    auto launch = [&](const LayerArgs &a) {
+        const unsigned grid = (unsigned)((u64)1 << (n - a.tb));
+        if (nt) sv_layer_kernel<S, 1><<<grid, SV_THREADS, 0, st>>>((S *)psi, a);
+        else sv_layer_kernel<S, 0><<<grid, SV_THREADS, 0, st>>>((S *)psi, a);
+        ORK_LAUNCH_CHECK();
+        return 0;
+    };
+    LayerArgs a{};
+    const int tb0 = n < LAYER_LT ? n : LAYER_LT;
+    a.tb = a.r = a.lo = tb0;
+    a.g = 0;
+    a.b0 = 0;
+    a.b1 = tb0;
+    for (int q = 0; q < tb0; ++q)
+        for (int i = 0; i < 8; ++i) a.m[q][i] = mats[8 * q + i];
+    int e = launch(a);
+    for (int lo = tb0; lo < n && !e; lo += LAYER_LT - LAYER_R) {
+        const int g = (n - lo) < (LAYER_LT - LAYER_R) ? (n - lo) : (LAYER_LT - LAYER_R);
+        LayerArgs b{};
+        b.r = LAYER_R;
+        b.g = g;
+        b.tb = LAYER_R + g;
+        b.lo = lo;
+        b.b0 = LAYER_R;
+        b.b1 = LAYER_R + g;
+        for (int j = 0; j < g; ++j)
+            for (int i = 0; i < 8; ++i) b.m[LAYER_R + j][i] = mats[8 * (lo + j) + i];
+        e = launch(b);
+    }
+    return e;
+}
+
 }  // namespace
 
 // ---- C ABI --------------------------------------------------------------------------------
 extern "C" uint64_t ork_partials_len(void) { return 2 * RED_BLOCKS + 2; }
+
+// mats: n x 8 doubles, the row-major 2x2 matrix of each qubit (re, im interleaved)
+extern "C" int ork_sv_layer(void *stream, int prec, void *psi, int n, const double *mats) {
+    if (n < 1 || n > 40 || !mats) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    return prec == ORK_F32 ? sv_layer<float2>(st, psi, n, mats) : sv_layer<double2>(st, psi, n, mats);
+}
 
 extern "C" int ork_sv_1q(void *stream, int prec, void *psi, int n, int q, int op, const double *coef) {
     Coef k;

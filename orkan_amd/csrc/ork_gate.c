@@ -308,3 +308,24 @@ void ccx(state_t *st, const qubit_t c1, const qubit_t c2, const qubit_t t) {
     GV(ork_type_ok(st), "ccx: unknown state type");
     g_ccx(st, c1, c2, t);
 }
+
+/* ---- fused layer of single-qubit gates (no reference counterpart) ---------------------------- */
+qs_error_t orkan_layer_1q(state_t *s, const cplx_t *mats) {
+    if (!s || !s->data || !mats) return QS_ERR_NULL;
+    if (!ork_type_ok(s) || ork_layout(s) != PURE) return QS_ERR_TYPE;
+    const int n = s->qubits;
+    double *m = malloc((size_t)8 * (n ? n : 1) * sizeof(double));
+    if (!m) return QS_ERR_OOM;
+    for (int q = 0; q < n; ++q) {
+        const cplx_t *c = mats + 4 * q;   /* column-major: c[0] = m00, c[1] = m10, c[2] = m01, c[3] = m11 */
+        const cplx_t rm[4] = {c[0], c[2], c[1], c[3]};
+        for (int i = 0; i < 4; ++i) {
+            m[8 * q + 2 * i] = creal(rm[i]);
+            m[8 * q + 2 * i + 1] = cimag(rm[i]);
+        }
+    }
+    const int e = n ? ork_sv_layer(stream(), ork_prec(s), s->data, n, m) : 0;
+    free(m);
+    ORK_HIP("orkan_layer_1q", e);
+    return QS_OK;
+}

@@ -61,6 +61,8 @@ int ork_fill(void *stream, int prec, void *data_dev, uint64_t len, double re, do
 int ork_fill_tiled_corner(void *stream, int prec, void *data_dev, int n, int log_tile, double v);
 
 int ork_sv_1q(void *stream, int prec, void *psi_dev, int n, int q, int op, const double *coef);
+/* every qubit's 2x2 matrix (mats: n x 8 doubles, row-major, re/im) in LDS-tiled sweeps */
+int ork_sv_layer(void *stream, int prec, void *psi_dev, int n, const double *mats);
 int ork_sv_mono(void *stream, int prec, void *psi_dev, int n, const ork_mono_t *m);
 int ork_sv_exp_diag(void *stream, int prec, void *psi_dev, int n, const double *diag_dev, double t);
 int ork_sv_mean(void *stream, int prec, const void *psi_dev, int n, const double *obs_dev,
