@@ -52,7 +52,7 @@ qs_error_t orkan_state_load(state_t *state, const char *path);
  * figures).  Applies mats[4 q .. 4 q + 3], the 2x2 matrix of qubit q in column-major order (the
  * single_from_mat convention), to every qubit q of a PURE state: the same state as calling the gates
  * one by one (they act on distinct qubits and commute), to rounding.  The statevector moves through
- * HBM in 1 + ceil((n - 12) / 8) LDS-tiled sweeps instead of n passes.  QS_ERR_NULL, QS_ERR_TYPE (not
+ * HBM in 1 + ceil((n - 12) / 8) LDS-tiled sweeps (default tile) instead of n passes.  QS_ERR_NULL, QS_ERR_TYPE (not
  * PURE).
  * ---------------------------------------------------------------------------------------------- */
 qs_error_t orkan_layer_1q(state_t *state, const cplx_t *mats);
@@ -103,7 +103,9 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * exp_diag, bit 3 mean / matel_diag; 0, the default, leaves them on where the tuned shapes use
  * them), 14: every
  * launch shape and tuning choice above back to the built-in defaults (flags and lds_bytes unused),
- * 15 / 16: as 6 / 7 for complex64 storage (kernels 6 and 7 then apply to complex128 only));
+ * 15 / 16: as 6 / 7 for complex64 storage (kernels 6 and 7 then apply to complex128 only),
+ * 17: tile and workgroup size of the orkan_layer_1q sweeps (flags only: 0 2^12 amplitudes / 256
+ * threads, 1 2^12 / 512, the default, 2 2^11 / 256, 3 2^11 / 512));
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9) give log2 of the workgroups that share one tile quad

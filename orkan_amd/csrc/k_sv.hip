@@ -460,7 +460,7 @@ int sv_1q(hipStream_t st, void *psi, int n, int q, int op, const Coef &k) {
 // so a tile holds 2^r consecutive amplitudes (whole lines) times the 2^g values of qubits
 // lo..lo+g-1; the first sweep (r = tb, g = 0) takes qubits 0..tb-1, each later one the next g
 // qubits, so every qubit's matrix is applied exactly once.
-constexpr int LAYER_LT = 12, LAYER_R = 4;
+constexpr int LAYER_LT = 12, LAYER_R = 4;   // largest tile (2^12 amplitudes) and the low run (16)
 struct LayerArgs {
     double m[LAYER_LT][8];   // per tile bit: m00, m01, m10, m11 (re, im), row-major
     int tb, r, lo, g;        // tile bits; low run bits; global bit of tile bit r; gate bits above the run
@@ -469,22 +469,22 @@ struct LayerArgs {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }   // one spare slot per 8 amplitudes
 
-template <typename S, int NT>
-__global__ __launch_bounds__(SV_THREADS) void sv_layer_kernel(S *__restrict__ psi, LayerArgs a) {
-    __shared__ cd tile[(1 << LAYER_LT) + ((1 << LAYER_LT) >> 3)];
+template <typename S, int NT, int LT, int THREADS>
+__global__ __launch_bounds__(THREADS) void sv_layer_kernel(S *__restrict__ psi, LayerArgs a) {
+    __shared__ cd tile[(1 << LT) + ((1 << LT) >> 3)];
     const int T = 1 << a.tb;
     const u64 t = blockIdx.x;
     const int midb = a.lo - a.r;
     const u64 tmid = t & (((u64)1 << midb) - 1), thigh = t >> midb;
     const u64 base = (tmid << a.r) | (thigh << (a.lo + a.g));
 #define LIDX(e) (base | (u64)((e) & ((1 << a.r) - 1)) | ((u64)((e) >> a.r) << a.lo))
-    for (int e = threadIdx.x; e < T; e += SV_THREADS) tile[lpad(e)] = ldx<NT>(psi + LIDX(e));
+    for (int e = threadIdx.x; e < T; e += THREADS) tile[lpad(e)] = ldx<NT>(psi + LIDX(e));
     __syncthreads();
     // up to three tile bits per pass: each thread holds the 8 amplitudes of one 3-bit group
     for (int b = a.b0; b < a.b1;) {
         const int rb = (a.b1 - b) < 3 ? (a.b1 - b) : 3;
         const int R = 1 << rb, ngroups = T >> rb, lowmask = (1 << b) - 1;
-        for (int gi = threadIdx.x; gi < ngroups; gi += SV_THREADS) {
+        for (int gi = threadIdx.x; gi < ngroups; gi += THREADS) {
             const int g0 = ((gi & ~lowmask) << rb) | (gi & lowmask);
             cd v[8];
 #pragma unroll
@@ -512,22 +512,39 @@ __global__ __launch_bounds__(SV_THREADS) void sv_layer_kernel(S *__restrict__ ps
         __syncthreads();
         b += rb;
     }
-    for (int e = threadIdx.x; e < T; e += SV_THREADS) stx<NT>(psi + LIDX(e), tile[lpad(e)]);
+    for (int e = threadIdx.x; e < T; e += THREADS) stx<NT>(psi + LIDX(e), tile[lpad(e)]);
 #undef LIDX
+}
+
+// tile bits and workgroup size of the layer sweeps: 0 (12, 256), 1 (12, 512), 2 (11, 256), 3 (11, 512).
+// Measured on 2^28 amplitudes: 7.9 / 6.0 / 10.3 / 10.1 ms per layer (complex128), 7.1 / 5.0 / 9.4 / 9.2
+// (complex64): the fourth wave per SIMD hides the LDS passes behind the other workgroup's HBM phases.
+constexpr int kLayerVariant = 1;
+static int g_layer_variant = kLayerVariant;
+
+template <typename S, int LT, int THREADS>
+void launch_layer(hipStream_t st, void *psi, int n, bool nt, const LayerArgs &a) {
+    const unsigned grid = (unsigned)((u64)1 << (n - a.tb));
+    if (nt) sv_layer_kernel<S, 1, LT, THREADS><<<grid, THREADS, 0, st>>>((S *)psi, a);
+    else sv_layer_kernel<S, 0, LT, THREADS><<<grid, THREADS, 0, st>>>((S *)psi, a);
 }
 
 template <typename S>
 int sv_layer(hipStream_t st, void *psi, int n, const double *mats) {
     const bool nt = !(g_nt_off & 1) && ((u64)sizeof(S) << n) >= g_nt_min_bytes;
+    const int v = g_layer_variant, LTv = v >= 2 ? 11 : 12;
     auto launch = [&](const LayerArgs &a) {
-        const unsigned grid = (unsigned)((u64)1 << (n - a.tb));
-        if (nt) sv_layer_kernel<S, 1><<<grid, SV_THREADS, 0, st>>>((S *)psi, a);
-        else sv_layer_kernel<S, 0><<<grid, SV_THREADS, 0, st>>>((S *)psi, a);
+        switch (v) {
+        case 1: launch_layer<S, 12, 512>(st, psi, n, nt, a); break;
+        case 2: launch_layer<S, 11, 256>(st, psi, n, nt, a); break;
+        case 3: launch_layer<S, 11, 512>(st, psi, n, nt, a); break;
+        default: launch_layer<S, 12, 256>(st, psi, n, nt, a); break;
+        }
         ORK_LAUNCH_CHECK();
         return 0;
     };
     LayerArgs a{};
-    const int tb0 = n < LAYER_LT ? n : LAYER_LT;
+    const int tb0 = n < LTv ? n : LTv;
     a.tb = a.r = a.lo = tb0;
     a.g = 0;
     a.b0 = 0;
@@ -535,8 +552,8 @@ int sv_layer(hipStream_t st, void *psi, int n, const double *mats) {
     for (int q = 0; q < tb0; ++q)
         for (int i = 0; i < 8; ++i) a.m[q][i] = mats[8 * q + i];
     int e = launch(a);
-    for (int lo = tb0; lo < n && !e; lo += LAYER_LT - LAYER_R) {
-        const int g = (n - lo) < (LAYER_LT - LAYER_R) ? (n - lo) : (LAYER_LT - LAYER_R);
+    for (int lo = tb0; lo < n && !e; lo += LTv - LAYER_R) {
+        const int g = (n - lo) < (LTv - LAYER_R) ? (n - lo) : (LTv - LAYER_R);
         LayerArgs b{};
         b.r = LAYER_R;
         b.g = g;
@@ -644,6 +661,12 @@ extern "C" int ork_sv_set_pair_shape(int flags, int lds) {
 
 // tuning hook: nontemporal accesses off per kernel family (bit 0 pair kernels, bit 1 permutations,
 // bit 2 exp_diag, bit 3 reductions); min_bytes: smallest state that uses them (0: 512 MiB)
+extern "C" int ork_sv_set_layer_variant(int v) {
+    if (v < 0 || v > 3) return (int)hipErrorInvalidValue;
+    g_layer_variant = v;
+    return 0;
+}
+
 extern "C" int ork_sv_set_nt_off(int mask, int min_bytes) {
     if (mask < 0 || mask > 15 || min_bytes < 0) return (int)hipErrorInvalidValue;
     g_nt_off = mask;
@@ -657,6 +680,7 @@ extern "C" void ork_sv_reset_tuning(void) {
     g_pair_override = SvShape{0, 0, 0, 0};
     g_nt_off = 0;
     g_nt_min_bytes = kNtMinBytes;
+    g_layer_variant = kLayerVariant;
     g_red_lds = kRedLds;
 }
 

@@ -90,6 +90,7 @@ int ork_sv_set_red_lds(int lds_bytes);
 int ork_mlp_gemm_select(int variant);
 int ork_sv_set_pair_shape(int flags, int lds_bytes);
 int ork_sv_set_nt_off(int mask, int min_bytes);
+int ork_sv_set_layer_variant(int v);
 unsigned long long ork_nt_min_bytes(void);
 void ork_dm_reset_shapes(void);
 void ork_sv_reset_tuning(void);

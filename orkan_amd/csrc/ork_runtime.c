@@ -102,6 +102,7 @@ int orkan_set_launch_shape(int kernel, int flags, int lds_bytes) {
     if (kernel == 11) return ork_mlp_gemm_select(flags);
     if (kernel == 12) return ork_sv_set_pair_shape(flags, lds_bytes);
     if (kernel == 13) return ork_sv_set_nt_off(flags, lds_bytes);
+    if (kernel == 17) return ork_sv_set_layer_variant(flags);
     if (kernel == 14) {
         ork_dm_reset_shapes();
         ork_sv_reset_tuning();
