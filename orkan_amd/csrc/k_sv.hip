@@ -469,6 +469,25 @@ struct LayerArgs {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }   // one spare slot per 8 amplitudes
 
+// the matrices m[0..rb) on bits 0..rb-1 of the index of v, ascending
+__device__ __forceinline__ void mat3(cd (&v)[8], const double (*m)[8], int rb) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= rb) break;
+        const cd m00 = cmk(m[k][0], m[k][1]), m01 = cmk(m[k][2], m[k][3]), m10 = cmk(m[k][4], m[k][5]),
+                 m11 = cmk(m[k][6], m[k][7]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (!((j >> k) & 1)) {
+                cd x0 = cmul(m00, v[j]), x1 = cmul(m10, v[j]);
+                cfma(x0, m01, v[j | (1 << k)]);
+                cfma(x1, m11, v[j | (1 << k)]);
+                v[j] = x0;
+                v[j | (1 << k)] = x1;
+            }
+    }
+}
+
 template <typename S, int NT, int LT, int THREADS>
 __global__ __launch_bounds__(THREADS) void sv_layer_kernel(S *__restrict__ psi, LayerArgs a) {
     __shared__ cd tile[(1 << LT) + ((1 << LT) >> 3)];
@@ -478,6 +497,45 @@ __global__ __launch_bounds__(THREADS) void sv_layer_kernel(S *__restrict__ psi, 
     const u64 tmid = t & (((u64)1 << midb) - 1), thigh = t >> midb;
     const u64 base = (tmid << a.r) | (thigh << (a.lo + a.g));
 #define LIDX(e) (base | (u64)((e) & ((1 << a.r) - 1)) | ((u64)((e) >> a.r) << a.lo))
+    if (T == 8 * THREADS && a.b1 == a.tb && a.b1 - a.b0 >= 3) {
+        // 8 amplitudes per thread, e = tid + u 2^UB: the top three tile bits are the register index
+        // u, so their matrices apply as the tile is loaded; the last LDS pass stores to HBM directly
+        const int UB = a.tb - 3;
+        cd v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ldx<NT>(psi + LIDX(threadIdx.x + (u << UB)));
+        mat3(v, a.m + UB, 3);
+        if (a.b0 == UB) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) stx<NT>(psi + LIDX(threadIdx.x + (u << UB)), v[u]);
+            return;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) tile[lpad(threadIdx.x + (u << UB))] = v[u];
+        __syncthreads();
+        for (int b = a.b0; b < UB;) {
+            const int rb = (UB - b) < 3 ? (UB - b) : 3;
+            const bool lastp = b + rb == UB;
+            const int R = 1 << rb, ngroups = T >> rb, lowmask = (1 << b) - 1;
+            for (int gi = threadIdx.x; gi < ngroups; gi += THREADS) {
+                const int g0 = ((gi & ~lowmask) << rb) | (gi & lowmask);
+                cd w[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < R) w[j] = tile[lpad(g0 | (j << b))];
+                mat3(w, a.m + b, rb);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < R) {
+                        if (lastp) stx<NT>(psi + LIDX(g0 | (j << b)), w[j]);
+                        else tile[lpad(g0 | (j << b))] = w[j];
+                    }
+            }
+            if (!lastp) __syncthreads();
+            b += rb;
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < T; e += THREADS) tile[lpad(e)] = ldx<NT>(psi + LIDX(e));
     __syncthreads();
     // up to three tile bits per pass: each thread holds the 8 amplitudes of one 3-bit group
@@ -517,8 +575,8 @@ __global__ __launch_bounds__(THREADS) void sv_layer_kernel(S *__restrict__ psi, 
 }
 
 // tile bits and workgroup size of the layer sweeps: 0 (12, 256), 1 (12, 512), 2 (11, 256), 3 (11, 512).
-// Measured on 2^28 amplitudes: 7.9 / 6.0 / 10.3 / 10.1 ms per layer (complex128), 7.1 / 5.0 / 9.4 / 9.2
-// (complex64): the fourth wave per SIMD hides the LDS passes behind the other workgroup's HBM phases.
+// Measured on 2^28 amplitudes: 8.2 / 4.9 / 10.0 / 15.3 ms per layer (complex128), 7.2 / 4.1 / 9.1 / 14.6
+// (complex64): 512 threads hold the tile's 8 amplitudes each, the register-resident path.
 constexpr int kLayerVariant = 1;
 static int g_layer_variant = kLayerVariant;
 
