@@ -110,7 +110,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9) give log2 of the workgroups that share one tile quad
  * or packed block (0..2), bit 3 of kernels 6, 15 (with bit 2) selects the lane-exchange kernel for
- * in-tile targets whose quad columns share 128 B lines;
+ * in-tile targets whose quad columns share 128 B lines, bit 5 hands each XCD runs of 32 consecutive
+ * groups instead of the contiguous ranges of bit 0;
  * lds_bytes caps workgroups per CU (0: no cap).  The settings are process-wide and unsynchronised:
  * change them while no other thread is calling into the library.  Returns 0, or hipErrorInvalidValue
  * for a kernel id or value outside the ranges above. */

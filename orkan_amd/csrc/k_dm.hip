@@ -69,8 +69,18 @@ __device__ __forceinline__ int gather_bits(u64 x, const int *pos, int k) {
 
 // with a grid of one workgroup per group, blocks are dispatched round-robin over
 // the 8 XCDs; the remap hands consecutive groups (neighbouring tiles) to one XCD
+// (xcd 1); xcd 2 hands each XCD runs of 32 consecutive groups in turn, so that neighbouring groups
+// mostly share an XCD while all XCDs still work on nearby groups at any time
 __device__ __forceinline__ u64 remap_group(u64 w, u64 n, int xcd) {
     if (!xcd || gridDim.x < n) return w;
+    if (xcd == 2) {
+        constexpr u64 G = 32, SPAN = 8 * G;
+        if (w < (n / SPAN) * SPAN) {
+            const u64 x = w & 7, k = w >> 3;
+            return (k / G) * SPAN + x * G + (k % G);
+        }
+        return w;
+    }
     const u64 per = n >> 3;
     if (w < (per << 3)) return (w & 7) * per + (w >> 3);
     return w;
@@ -883,10 +893,11 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
 }
 
 // streamed packed superoperator: column order and an 80 KB residency cap; each 32 x 32 block split
-// over 4 workgroups of 8 columns (16 KB each), dispatched in order.  Measured on 14 qubits: 5.24
-// TB/s (whole blocks with the XCD remap 5.06, the tile groups 4.7).  Complex64 storage keeps whole
+// over 4 workgroups of 8 columns (16 KB each), each XCD taking runs of 32 workgroups (8 blocks) in
+// turn.  Measured on 14 qubits: 5.37 TB/s (plain dispatch order 5.24, whole blocks with contiguous
+// XCD ranges 5.06, the tile groups 4.7).  Complex64 storage keeps whole
 // blocks (see launch_pstream).
-constexpr Shape kPstreamShape = {0, 1, 81920, 0, 2};
+constexpr Shape kPstreamShape = {2, 1, 81920, 0, 2};
 Shape g_pstream_shape = kPstreamShape;
 
 template <typename S>
@@ -985,9 +996,10 @@ extern "C" void ork_dm_reset_shapes(void) {
 }
 
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 31 || (flags >> 3) > 2)
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 63 || ((flags >> 3) & 3) > 2)
         return (int)hipErrorInvalidValue;
-    const Shape sh{flags & 1, (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, flags >> 3};
+    // bit 5: XCD runs of 32 groups instead of contiguous ranges
+    const Shape sh{(flags & 32) ? 2 : (flags & 1), (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, (flags >> 3) & 3};
     switch (which) {
     case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
     case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;
