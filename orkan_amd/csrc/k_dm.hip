@@ -790,8 +790,9 @@ struct Shape { int xcd, col, lds, nt, lp; };
 constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 0}};   // [packed, tiled]
 // phase kernels (cz, exp_diag, ...) measured on 14 qubits: packed best in column order, tiled in plain
 // order with nontemporal accesses where the changed entries fill whole 128 B lines (cz from qubits
-// (3, 4) up: +4 % over the neighbour pairs; on the packed layout they lose 5-15 %)
-constexpr Shape kPhaseShape[2] = {{0, 1, 0, 0}, {0, 0, 0, 1}};
+// (3, 4) up: +4 % over the neighbour pairs; on the packed layout they lose 5-15 %); both with XCD runs
+// of 32 groups (+2 %)
+constexpr Shape kPhaseShape[2] = {{2, 1, 0, 0}, {2, 0, 0, 1}};
 Shape g_group_shape[2] = {kGroupShape[0], kGroupShape[1]};
 Shape g_phase_shape[2] = {kPhaseShape[0], kPhaseShape[1]};
 
