@@ -13,7 +13,7 @@ namespace {
 constexpr int SV_THREADS = 256;
 constexpr int SV_UNROLL = 2;      // bases per thread in the permutation kernel (doubled for complex64)
 constexpr int RED_THREADS = 256;
-constexpr int RED_BLOCKS = 2048;  // first-stage partial sums (fixed: deterministic order)
+constexpr int RED_BLOCKS = 2048;  // room for first-stage partials; their count depends only on the size and shape
 
 struct Coef {
     double v[8];
@@ -297,44 +297,54 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
-// Four independent accumulators per thread: x, x + G, x + 2G, x + 3G (G = grid stride) are loaded
-// together, so each thread keeps four loads in flight; they are added in that fixed order.
-constexpr int RED_UNROLL = 4;
+// U independent accumulators per thread: x, x + G, ..., x + (U-1)G (G = grid stride) are loaded
+// together, so each thread keeps U loads in flight; they are added in that fixed order.
+template <int U>
+__device__ __forceinline__ double sum_fixed(const double (&v)[U]) {
+    double s = v[0];
+#pragma unroll
+    for (int u = 1; u < U; ++u) s += v[u];
+    return s;
+}
 
-template <typename S, int NT>
+template <typename S, int NT, int U>
 __global__ __launch_bounds__(RED_THREADS) void sv_mean_kernel(const S *__restrict__ psi, const double *__restrict__ obs,
                                                               u64 dim, double *__restrict__ part) {
-    double a4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0};
-    const u64 G = (u64)gridDim.x * RED_THREADS;
-    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * RED_UNROLL) {
-        cd a[RED_UNROLL];
-        double o[RED_UNROLL];
+    double a4[U];
 #pragma unroll
-        for (int u = 0; u < RED_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) a4[u] = 0.0;
+    const u64 G = (u64)gridDim.x * RED_THREADS;
+    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * U) {
+        cd a[U];
+        double o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
             const u64 x = x0 + (u64)u * G;
             const bool in = x < dim;
             a[u] = in ? ldx<NT>(psi + x) : cmk(0.0, 0.0);
             o[u] = in ? (NT ? __builtin_nontemporal_load(obs + x) : obs[x]) : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < RED_UNROLL; ++u) a4[u] += o[u] * (a[u].x * a[u].x + a[u].y * a[u].y);
+        for (int u = 0; u < U; ++u) a4[u] += o[u] * (a[u].x * a[u].x + a[u].y * a[u].y);
     }
-    double acc[1] = {((a4[0] + a4[1]) + a4[2]) + a4[3]};
+    double acc[1] = {sum_fixed<U>(a4)};
     block_sum<1>(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-template <typename S, int NT>
+template <typename S, int NT, int U>
 __global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restrict__ bra, const S *__restrict__ ket,
                                                                const double *__restrict__ obs, u64 dim,
                                                                double *__restrict__ part) {
-    double re4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0}, im4[RED_UNROLL] = {0.0, 0.0, 0.0, 0.0};
-    const u64 G = (u64)gridDim.x * RED_THREADS;
-    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * RED_UNROLL) {
-        cd a[RED_UNROLL], b[RED_UNROLL];
-        double o[RED_UNROLL];
+    double re4[U], im4[U];
 #pragma unroll
-        for (int u = 0; u < RED_UNROLL; ++u) {
+    for (int u = 0; u < U; ++u) re4[u] = im4[u] = 0.0;
+    const u64 G = (u64)gridDim.x * RED_THREADS;
+    for (u64 x0 = (u64)blockIdx.x * RED_THREADS + threadIdx.x; x0 < dim; x0 += G * U) {
+        cd a[U], b[U];
+        double o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
             const u64 x = x0 + (u64)u * G;
             const bool in = x < dim;
             a[u] = in ? ldx<NT>(bra + x) : cmk(0.0, 0.0);
@@ -342,12 +352,12 @@ __global__ __launch_bounds__(RED_THREADS) void sv_matel_kernel(const S *__restri
             o[u] = in ? (NT ? __builtin_nontemporal_load(obs + x) : obs[x]) : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < RED_UNROLL; ++u) {
+        for (int u = 0; u < U; ++u) {
             re4[u] += o[u] * (a[u].x * b[u].x + a[u].y * b[u].y);
             im4[u] += o[u] * (a[u].x * b[u].y - a[u].y * b[u].x);
         }
     }
-    double acc[2] = {((re4[0] + re4[1]) + re4[2]) + re4[3], ((im4[0] + im4[1]) + im4[2]) + im4[3]};
+    double acc[2] = {sum_fixed<U>(re4), sum_fixed<U>(im4)};
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = acc[0];
@@ -705,9 +715,16 @@ extern "C" int ork_sv_exp_diag(void *stream, int prec, void *psi, int n, const d
     return 0;
 }
 
-// residency cap (dynamic LDS bytes per workgroup) of the mean / matel first stage
-constexpr int kRedLds = 81920;   // measured best on 2^28 amplitudes (5.33 TB/s against 5.21 uncapped)
+// First-stage shape of the mean / matel reductions: dynamic LDS bytes per workgroup (a residency
+// cap), loads in flight per thread (4 or 8) and the largest number of workgroups (RED_BLOCKS >> k).
+// Measured on 2^28 amplitudes, host read included: 256 workgroups (one per CU: 96 KB of LDS each
+// leaves room for no second one) with 4 loads in flight read 6.94 TB/s (mean) and 7.04 TB/s
+// (matel_diag); 2048 workgroups grid-striding in four rounds of resident groups reached 5.85 / 5.94,
+// 512 reached 6.74 / 6.17, and 128 ran out of loads in flight (4.13 / 5.60).
+constexpr int kRedLds = 98304;
 static int g_red_lds = kRedLds;
+constexpr int kRedUnroll = 4, kRedShift = 3;
+static int g_red_unroll = kRedUnroll, g_red_shift = kRedShift;
 // flags bit 4 on: every target qubit uses (bit 0 xcd, bit 1 nontemporal, bits 2-3 log2 of the pairs
 // per thread: 0..2) with LDS cap lds; bit 4 off: back to the per-qubit table
 extern "C" int ork_sv_set_pair_shape(int flags, int lds) {
@@ -740,19 +757,42 @@ extern "C" void ork_sv_reset_tuning(void) {
     g_nt_min_bytes = kNtMinBytes;
     g_layer_variant = kLayerVariant;
     g_red_lds = kRedLds;
+    g_red_unroll = kRedUnroll;
+    g_red_shift = kRedShift;
 }
 
-extern "C" int ork_sv_set_red_lds(int lds) {
-    if (lds < 0 || lds > 160 * 1024) return (int)hipErrorInvalidValue;
+// flags bit 0: 8 loads in flight per thread (else 4); bits 1-3: k, at most RED_BLOCKS >> k workgroups
+extern "C" int ork_sv_set_red_lds(int flags, int lds) {
+    if (lds < 0 || lds > 160 * 1024 || flags < 0 || flags > 9) return (int)hipErrorInvalidValue;
     g_red_lds = lds;
+    g_red_unroll = (flags & 1) ? 8 : 4;
+    g_red_shift = flags >> 1;
     return 0;
 }
 
 static int red_blocks(u64 dim) {
+    const u64 cap = (u64)(RED_BLOCKS >> g_red_shift);
     u64 b = (dim + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
     if (b < 1) b = 1;
-    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    if (b > cap) b = cap;
     return (int)b;
+}
+
+template <typename S, int NT>
+static void launch_mean(hipStream_t st, int nb, const void *psi, const double *obs, u64 dim, double *part) {
+    if (g_red_unroll == 8)
+        sv_mean_kernel<S, NT, 8><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)psi, obs, dim, part);
+    else
+        sv_mean_kernel<S, NT, 4><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)psi, obs, dim, part);
+}
+
+template <typename S, int NT>
+static void launch_matel(hipStream_t st, int nb, const void *bra, const void *ket, const double *obs, u64 dim,
+                         double *part) {
+    if (g_red_unroll == 8)
+        sv_matel_kernel<S, NT, 8><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
+    else
+        sv_matel_kernel<S, NT, 4><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
 }
 
 extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const double *obs, double *part,
@@ -762,11 +802,11 @@ extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const
     hipStream_t st = (hipStream_t)stream;
     const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
-        if (nt) sv_mean_kernel<float2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)psi, obs, dim, part);
-        else sv_mean_kernel<float2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)psi, obs, dim, part);
+        if (nt) launch_mean<float2, 1>(st, nb, psi, obs, dim, part);
+        else launch_mean<float2, 0>(st, nb, psi, obs, dim, part);
     } else {
-        if (nt) sv_mean_kernel<double2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)psi, obs, dim, part);
-        else sv_mean_kernel<double2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)psi, obs, dim, part);
+        if (nt) launch_mean<double2, 1>(st, nb, psi, obs, dim, part);
+        else launch_mean<double2, 0>(st, nb, psi, obs, dim, part);
     }
     ORK_LAUNCH_CHECK();
     finish_kernel<1><<<1, RED_THREADS, 0, st>>>(part, nb);
@@ -783,19 +823,11 @@ extern "C" int ork_sv_matel(void *stream, int prec, const void *bra, const void 
     hipStream_t st = (hipStream_t)stream;
     const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
-        if (nt)
-            sv_matel_kernel<float2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)bra, (const float2 *)ket, obs,
-                                                                           dim, part);
-        else
-            sv_matel_kernel<float2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const float2 *)bra, (const float2 *)ket, obs,
-                                                                           dim, part);
+        if (nt) launch_matel<float2, 1>(st, nb, bra, ket, obs, dim, part);
+        else launch_matel<float2, 0>(st, nb, bra, ket, obs, dim, part);
     } else {
-        if (nt)
-            sv_matel_kernel<double2, 1><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)bra, (const double2 *)ket,
-                                                                            obs, dim, part);
-        else
-            sv_matel_kernel<double2, 0><<<nb, RED_THREADS, g_red_lds, st>>>((const double2 *)bra, (const double2 *)ket,
-                                                                            obs, dim, part);
+        if (nt) launch_matel<double2, 1>(st, nb, bra, ket, obs, dim, part);
+        else launch_matel<double2, 0>(st, nb, bra, ket, obs, dim, part);
     }
     ORK_LAUNCH_CHECK();
     finish_kernel<2><<<1, RED_THREADS, 0, st>>>(part, nb);

@@ -98,7 +98,7 @@ int orkan_dev_upload(double *dst_dev, const double *src_host, idx_t count) {
 }
 
 int orkan_set_launch_shape(int kernel, int flags, int lds_bytes) {
-    if (kernel == 10) return ork_sv_set_red_lds(lds_bytes);
+    if (kernel == 10) return ork_sv_set_red_lds(flags, lds_bytes);
     if (kernel == 11) return ork_mlp_gemm_select(flags);
     if (kernel == 12) return ork_sv_set_pair_shape(flags, lds_bytes);
     if (kernel == 13) return ork_sv_set_nt_off(flags, lds_bytes);
