@@ -48,8 +48,8 @@ qs_error_t orkan_state_load(state_t *state, const char *path);
 
 /* ------------------------------------------------------------------------------------------------
  * Fused layer of single-qubit gates (no counterpart in the reference, which applies one gate per
- * call; the reference's survey of next steps names gate fusion, reported apart from the per-gate
- * figures).  Applies mats[4 q .. 4 q + 3], the 2x2 matrix of qubit q in column-major order (the
+ * call; an addition for circuits that apply a layer of rotations on every qubit, reported apart
+ * from the per-gate figures).  Applies mats[4 q .. 4 q + 3], the 2x2 matrix of qubit q in column-major order (the
  * single_from_mat convention), to every qubit q of a PURE state: the same state as calling the gates
  * one by one (they act on distinct qubits and commute), to rounding.  The statevector moves through
  * HBM in 1 + ceil((n - 12) / 8) LDS-tiled sweeps (default tile) instead of n passes.  QS_ERR_NULL, QS_ERR_TYPE (not

@@ -1,7 +1,7 @@
 // k_qaoa.hip -- batched QAOA evaluation: thousands of (graph, angles) circuits
 // of uneven qubit counts in one persistent launch.
 //
-// One evaluation is the reference's own call sequence (SURVEY call stack D):
+// One evaluation is the reference's own QAOA call sequence:
 //   state_plus(PURE, n); p x { exp_diag(C, gamma); rx(q, 2 beta) for q = 0..n-1 };
 //   mean(C)
 // (src/state/state_pure.c:30-43, src/circ/circ_pure.c:21-34,

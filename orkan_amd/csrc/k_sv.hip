@@ -734,14 +734,15 @@ extern "C" int ork_sv_set_pair_shape(int flags, int lds) {
     return 0;
 }
 
-// tuning hook: nontemporal accesses off per kernel family (bit 0 pair kernels, bit 1 permutations,
-// bit 2 exp_diag, bit 3 reductions); min_bytes: smallest state that uses them (0: 512 MiB)
+// tuning hook: tile bits and workgroup size of the fused-layer sweeps (variants 0-3)
 extern "C" int ork_sv_set_layer_variant(int v) {
     if (v < 0 || v > 3) return (int)hipErrorInvalidValue;
     g_layer_variant = v;
     return 0;
 }
 
+// tuning hook: nontemporal accesses off per kernel family (bit 0 pair kernels, bit 1 permutations,
+// bit 2 exp_diag, bit 3 reductions); min_bytes: smallest state that uses them (0: 512 MiB)
 extern "C" int ork_sv_set_nt_off(int mask, int min_bytes) {
     if (mask < 0 || mask > 15 || min_bytes < 0) return (int)hipErrorInvalidValue;
     g_nt_off = mask;
