@@ -900,6 +900,11 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
 // blocks (see launch_pstream).
 constexpr Shape kPstreamShape = {2, 1, 81920, 0, 2};
 Shape g_pstream_shape = kPstreamShape;
+// complex64: half a block per workgroup, XCD runs, at most 3 workgroups per CU.  Measured per
+// superoperator on 14 qubits: 5.22 TB/s against 4.12 for whole blocks with the contiguous XCD
+// remap and 81920 B (the shape kept before the runs existed), 5.11-5.14 for quarter blocks.
+constexpr Shape kPstreamShape32 = {2, 1, 40960, 0, 1};
+Shape g_pstream_shape32 = kPstreamShape32;
 
 template <typename S>
 int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
@@ -908,22 +913,16 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     pp.q = q;
     pp.nt = pp.dim >> 6;
     pp.nblocks = pp.nt * (pp.nt + 1) / 2;
-    pp.xcd = g_pstream_shape.xcd;
-    pp.col = g_pstream_shape.col;
+    const Shape &sh = sizeof(S) == 8 ? g_pstream_shape32 : g_pstream_shape;
+    pp.xcd = sh.xcd;
+    pp.col = sh.col;
     if (pp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-    const int lds = g_pstream_shape.lds > (int)sizeof(cd) * 32 * LDS_STRIDE
-                        ? g_pstream_shape.lds - (int)sizeof(cd) * 32 * LDS_STRIDE : 0;
-    int lp = g_pstream_shape.lp;
-    if (sizeof(S) == 8 && lp) {
-        // complex64: a quarter block per workgroup leaves too few bytes in flight (3.4 against 3.8
-        // TB/s for the whole layer); whole blocks with the XCD remap
-        lp = 0;
-        pp.xcd = 1;
-    }
+    const int lds = sh.lds > (int)sizeof(cd) * 32 * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * 32 * LDS_STRIDE : 0;
+    const int lp = sh.lp;
     if ((pp.nblocks << lp) > 0x7fffffffu) return (int)hipErrorInvalidValue;
     const unsigned nb = (unsigned)(pp.nblocks << lp);
     S *r = (S *)rho;
-    if (g_pstream_shape.nt) {
+    if (sh.nt) {
         if (lp == 2) dm_sop_packed_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
         else if (lp == 1) dm_sop_packed_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
         else dm_sop_packed_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
@@ -994,6 +993,7 @@ extern "C" void ork_dm_reset_shapes(void) {
     for (int i = 0; i < 4; ++i) g_stream_shape[i] = kStreamShape[i];
     g_stream_on = 1;
     g_pstream_shape = kPstreamShape;
+    g_pstream_shape32 = kPstreamShape32;
 }
 
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
@@ -1012,6 +1012,7 @@ extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
     case 7: g_stream_shape[1] = sh; break;
     case 8: g_stream_on = lds_bytes != 0; break;
     case 9: g_pstream_shape = sh; break;
+    case 18: g_pstream_shape32 = sh; break;
     case 15: g_stream_shape[2] = sh; break;
     case 16: g_stream_shape[3] = sh; break;
     default: return (int)hipErrorInvalidValue;

@@ -6,7 +6,7 @@ Statevector kernels use them on states of 512 MiB and more (2^25 amplitudes in
 complex128, 2^26 in complex64), where the tuned shapes switch them on; the hook
 (kernel 13) turns them off per family and sets that size.  The streamed
 density-matrix kernels and the phase kernels (cz, exp_diag) take them from
-bit 2 of their launch-shape flags (kernels 4-7, 9) on states of at least that
+bit 2 of their launch-shape flags (kernels 4-7, 9, 15, 16, 18) on states of at least that
 size, which the test lowers to one byte for a 10-qubit density matrix."""
 import ctypes as C
 
@@ -112,6 +112,7 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, eng3, lt, kind, 
             assert api.orkan_set_launch_shape(k6, 1 | 12 * nt, 65536) == 0
             assert api.orkan_set_launch_shape(k7, 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
+        assert api.orkan_set_launch_shape(18, 3 | 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(4, 3 | 4 * nt, 0) == 0
         assert api.orkan_set_launch_shape(5, 4 * nt, 0) == 0
         st = eng.new(kind, n, rho)
@@ -142,7 +143,7 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, eng3, lt, kind, 
 @pytest.mark.parametrize("fp32", [False, True])
 def test_split_tile_quads_change_no_value(eng, kind, fp32):
     """The streamed superoperator kernels with 2 or 4 workgroups per tile quad (tiled, target above
-    the tile) or per 32 x 32 block of quads (packed) -- bits 3-4 of the kernel-7 / 16 / 9 shape
+    the tile) or per 32 x 32 block of quads (packed) -- bits 3-4 of the kernel-7 / 16 / 9 / 18 shape
     flags -- form the same products for every entry as with one: bit-identical."""
     api = eng.api
     n = 9
@@ -156,6 +157,7 @@ def test_split_tile_quads_change_no_value(eng, kind, fp32):
         assert api.orkan_set_launch_shape(7, 8 * lp + 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(16, 8 * lp + 4 * nt, 81920) == 0
         assert api.orkan_set_launch_shape(9, 8 * lp + 4 * nt + 3, 81920) == 0
+        assert api.orkan_set_launch_shape(18, 8 * lp + 4 * nt + 3, 81920) == 0
         st = eng.new(kind, n, rho)
         if fp32:
             st32 = api.orkan_state_convert(C.byref(st), 1)
