@@ -180,3 +180,35 @@ def test_split_tile_quads_change_no_value(eng, kind, fp32):
     for lp in (1, 2):
         for nt in (0, 1):
             assert np.array_equal(run(lp, nt), ref), (lp, nt)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_exp_diag_shapes_change_no_value(eng, fp32):
+    """exp_diag with 1, 2 or 4 amplitudes per thread and any residency cap (tuning hook kernel 19)
+    forms the same product for every amplitude: bit-identical, on a size that is no multiple of a
+    workgroup's amplitudes (2^7 with 4 per thread)."""
+    api = eng.api
+    g = wl.rng(47)
+    outs = []
+    for n in (7, 15):
+        psi = (g.standard_normal(1 << n) + 1j * g.standard_normal(1 << n)) / np.sqrt(2 << n)
+        diag = g.uniform(-3, 3, 1 << n)
+        res = []
+        for lu, lds in ((0, 0), (1, 32768), (2, 32768), (2, 0)):
+            assert api.orkan_set_launch_shape(19, lu, lds) == 0
+            st = eng.new(PURE, n, psi)
+            if fp32:
+                st32 = api.orkan_state_convert(C.byref(st), 1)
+                eng.free(st)
+                st = st32
+            eng.exp_diag(st, diag, 0.9)
+            if fp32:
+                back = api.orkan_state_convert(C.byref(st), 0)
+                eng.free(st)
+                st = back
+            res.append(eng.read(st))
+            eng.free(st)
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert all(np.array_equal(r, res[0]) for r in res[1:])
+        outs.append(res[0])
+    assert len(outs) == 2
