@@ -260,24 +260,36 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
 template <typename S, int NT, int U>
 __global__ __launch_bounds__(SV_THREADS) void sv_exp_diag_kernel(S *__restrict__ psi, const double *__restrict__ diag,
                                                                  u64 dim, double t) {
-    const u64 base = (u64)blockIdx.x * (SV_THREADS * U) + threadIdx.x;
-    double d[U];
-    cd v[U];
+    if constexpr (U == 1) {
+        // one amplitude per thread: the diagonal entry's load, then its sincos while psi's load is
+        // in flight
+        const u64 x = (u64)blockIdx.x * SV_THREADS + threadIdx.x;
+        if (x >= dim) return;
+        double s, c;
+        const double d = NT ? __builtin_nontemporal_load(diag + x) : diag[x];
+        sincos(d * t, &s, &c);
+        const cd v = ldx<NT>(psi + x);
+        stx<NT>(psi + x, cmk(c * v.x + s * v.y, c * v.y - s * v.x));
+    } else {
+        const u64 base = (u64)blockIdx.x * (SV_THREADS * U) + threadIdx.x;
+        double d[U];
+        cd v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 x = base + (u64)u * SV_THREADS;
-        if (x < dim) {
-            d[u] = NT ? __builtin_nontemporal_load(diag + x) : diag[x];
-            v[u] = ldx<NT>(psi + x);
+        for (int u = 0; u < U; ++u) {
+            const u64 x = base + (u64)u * SV_THREADS;
+            if (x < dim) {
+                d[u] = NT ? __builtin_nontemporal_load(diag + x) : diag[x];
+                v[u] = ldx<NT>(psi + x);
+            }
         }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 x = base + (u64)u * SV_THREADS;
-        if (x < dim) {
-            double s, c;
-            sincos(d[u] * t, &s, &c);
-            stx<NT>(psi + x, cmk(c * v[u].x + s * v[u].y, c * v[u].y - s * v[u].x));
+        for (int u = 0; u < U; ++u) {
+            const u64 x = base + (u64)u * SV_THREADS;
+            if (x < dim) {
+                double s, c;
+                sincos(d[u] * t, &s, &c);
+                stx<NT>(psi + x, cmk(c * v[u].x + s * v[u].y, c * v[u].y - s * v[u].x));
+            }
         }
     }
 }
