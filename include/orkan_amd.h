@@ -92,7 +92,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * packed / tiled layout, 6 / 7: streamed tiled 1-qubit superoperator with the target inside / above
  * the tile, 8: streamed superoperator on (lds_bytes != 0) or off, in favour of the tile groups,
  * 9: streamed packed superoperator, 10: statevector mean / matel_diag reduction (flags bit 0: 8 loads in flight
- * per thread, else 4; bits 1-3: k, at most 2048 >> k workgroups, k <= 4),
+ * per thread, else 4; bits 1-3: k, at most 2048 >> k workgroups, k <= 4; bit 4: the shape of
+ * complex64 states, else of complex128 ones),
  * 11: surrogate GEMM tile where the shape allows (flags only: 0 128 x 128, 1 256 x 256 staged
  * through registers, 2 256 x 256 streamed into LDS by DMA, the default), 12: statevector 1-qubit
  * kernels for every target qubit (flags bit 4 on: bit 0 XCD remap, bit 1 nontemporal loads and

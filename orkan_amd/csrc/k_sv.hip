@@ -769,9 +769,15 @@ extern "C" int ork_sv_exp_diag(void *stream, int prec, void *psi, int n, const d
 // (matel_diag); 2048 workgroups grid-striding in four rounds of resident groups reached 5.85 / 5.94,
 // 512 reached 6.74 / 6.17, and 128 ran out of loads in flight (4.13 / 5.60).
 constexpr int kRedLds = 98304;
-static int g_red_lds = kRedLds;
 constexpr int kRedUnroll = 4, kRedShift = 3;
-static int g_red_unroll = kRedUnroll, g_red_shift = kRedShift;
+// complex64 storage: 8 B per amplitude leaves too few bytes in flight in one workgroup per CU
+// (2.78 / 2.93 TB/s); 1024 uncapped workgroups, all resident at once, read 6.86 / 6.88 TB/s
+constexpr int kRedLds32 = 0;
+constexpr int kRedUnroll32 = 4, kRedShift32 = 1;
+struct RedShape {
+    int lds, unroll, shift;
+};
+static RedShape g_red[2] = {{kRedLds, kRedUnroll, kRedShift}, {kRedLds32, kRedUnroll32, kRedShift32}};
 // flags bit 4 on: every target qubit uses (bit 0 xcd, bit 1 nontemporal, bits 2-3 log2 of the pairs
 // per thread: 0..2) with LDS cap lds; bit 4 off: back to the per-qubit table
 extern "C" int ork_sv_set_pair_shape(int flags, int lds) {
@@ -804,24 +810,22 @@ extern "C" void ork_sv_reset_tuning(void) {
     g_nt_off = 0;
     g_nt_min_bytes = kNtMinBytes;
     g_layer_variant = kLayerVariant;
-    g_red_lds = kRedLds;
-    g_red_unroll = kRedUnroll;
+    g_red[0] = RedShape{kRedLds, kRedUnroll, kRedShift};
+    g_red[1] = RedShape{kRedLds32, kRedUnroll32, kRedShift32};
     g_exp_u = kExpU;
     g_exp_lds = kExpLds;
-    g_red_shift = kRedShift;
 }
 
-// flags bit 0: 8 loads in flight per thread (else 4); bits 1-3: k, at most RED_BLOCKS >> k workgroups
+// flags bit 0: 8 loads in flight per thread (else 4); bits 1-3: k, at most RED_BLOCKS >> k workgroups;
+// bit 4: the complex64 shape (else complex128)
 extern "C" int ork_sv_set_red_lds(int flags, int lds) {
-    if (lds < 0 || lds > 160 * 1024 || flags < 0 || flags > 9) return (int)hipErrorInvalidValue;
-    g_red_lds = lds;
-    g_red_unroll = (flags & 1) ? 8 : 4;
-    g_red_shift = flags >> 1;
+    if (lds < 0 || lds > 160 * 1024 || flags < 0 || flags > 31 || (flags & 15) > 9) return (int)hipErrorInvalidValue;
+    g_red[flags >> 4] = RedShape{lds, (flags & 1) ? 8 : 4, (flags & 15) >> 1};
     return 0;
 }
 
-static int red_blocks(u64 dim) {
-    const u64 cap = (u64)(RED_BLOCKS >> g_red_shift);
+static int red_blocks(u64 dim, const RedShape &r) {
+    const u64 cap = (u64)(RED_BLOCKS >> r.shift);
     u64 b = (dim + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
     if (b < 1) b = 1;
     if (b > cap) b = cap;
@@ -830,25 +834,27 @@ static int red_blocks(u64 dim) {
 
 template <typename S, int NT>
 static void launch_mean(hipStream_t st, int nb, const void *psi, const double *obs, u64 dim, double *part) {
-    if (g_red_unroll == 8)
-        sv_mean_kernel<S, NT, 8><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)psi, obs, dim, part);
+    const RedShape &r = g_red[sizeof(S) == 8];
+    if (r.unroll == 8)
+        sv_mean_kernel<S, NT, 8><<<nb, RED_THREADS, r.lds, st>>>((const S *)psi, obs, dim, part);
     else
-        sv_mean_kernel<S, NT, 4><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)psi, obs, dim, part);
+        sv_mean_kernel<S, NT, 4><<<nb, RED_THREADS, r.lds, st>>>((const S *)psi, obs, dim, part);
 }
 
 template <typename S, int NT>
 static void launch_matel(hipStream_t st, int nb, const void *bra, const void *ket, const double *obs, u64 dim,
                          double *part) {
-    if (g_red_unroll == 8)
-        sv_matel_kernel<S, NT, 8><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
+    const RedShape &r = g_red[sizeof(S) == 8];
+    if (r.unroll == 8)
+        sv_matel_kernel<S, NT, 8><<<nb, RED_THREADS, r.lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
     else
-        sv_matel_kernel<S, NT, 4><<<nb, RED_THREADS, g_red_lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
+        sv_matel_kernel<S, NT, 4><<<nb, RED_THREADS, r.lds, st>>>((const S *)bra, (const S *)ket, obs, dim, part);
 }
 
 extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const double *obs, double *part,
                            double *out) {
     const u64 dim = (u64)1 << n;
-    const int nb = red_blocks(dim);
+    const int nb = red_blocks(dim, g_red[prec == ORK_F32]);
     hipStream_t st = (hipStream_t)stream;
     const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
@@ -869,7 +875,7 @@ extern "C" int ork_sv_mean(void *stream, int prec, const void *psi, int n, const
 extern "C" int ork_sv_matel(void *stream, int prec, const void *bra, const void *ket, int n, const double *obs,
                             double *part, double *out) {
     const u64 dim = (u64)1 << n;
-    const int nb = red_blocks(dim);
+    const int nb = red_blocks(dim, g_red[prec == ORK_F32]);
     hipStream_t st = (hipStream_t)stream;
     const bool nt = !(g_nt_off & 8) && dim * (prec == ORK_F32 ? 8 : 16) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
