@@ -108,8 +108,9 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * 15 / 16: as 6 / 7 for complex64 storage (kernels 6 and 7 then apply to complex128 only),
  * 17: tile and workgroup size of the orkan_layer_1q sweeps (flags only: 0 2^12 amplitudes / 256
  * threads, 1 2^12 / 512, the default, 2 2^11 / 256, 3 2^11 / 512)), 18: as 9 for complex64 storage
- * (kernel 9 then applies to complex128 only), 19: statevector exp_diag (flags: log2 of the amplitudes
- * per thread, 0..2; lds_bytes as below);
+ * (kernel 9 then applies to complex128 only), 19: statevector exp_diag (flags bits 0-1: log2 of the
+ * amplitudes per thread, 0..2, bit 2: the shape of complex64 states, else of complex128 ones;
+ * lds_bytes as below);
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad

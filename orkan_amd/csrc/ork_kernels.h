@@ -87,7 +87,7 @@ int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rh
 int ork_dm_set_shape(int which, int flags, int lds_bytes);
 /* residency cap (dynamic LDS bytes per workgroup) of the statevector mean / matel reduction */
 int ork_sv_set_red_lds(int flags, int lds_bytes);
-int ork_sv_set_exp_shape(int log2_u, int lds_bytes);
+int ork_sv_set_exp_shape(int flags, int lds_bytes);
 int ork_mlp_gemm_select(int variant);
 int ork_sv_set_pair_shape(int flags, int lds_bytes);
 int ork_sv_set_nt_off(int mask, int min_bytes);

@@ -195,7 +195,7 @@ def test_exp_diag_shapes_change_no_value(eng, fp32):
         diag = g.uniform(-3, 3, 1 << n)
         res = []
         for lu, lds in ((0, 0), (1, 32768), (2, 32768), (2, 0)):
-            assert api.orkan_set_launch_shape(19, lu, lds) == 0
+            assert api.orkan_set_launch_shape(19, 4 * fp32 + lu, lds) == 0
             st = eng.new(PURE, n, psi)
             if fp32:
                 st32 = api.orkan_state_convert(C.byref(st), 1)
