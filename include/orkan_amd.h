@@ -110,7 +110,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * threads, 1 2^12 / 512, the default, 2 2^11 / 256, 3 2^11 / 512)), 18: as 9 for complex64 storage
  * (kernel 9 then applies to complex128 only), 19: statevector exp_diag (flags bits 0-1: log2 of the
  * amplitudes per thread, 0..2, bit 2: the shape of complex64 states, else of complex128 ones;
- * lds_bytes as below);
+ * lds_bytes as below), 20: permutation / phase kernels (cx, cz, swap, ...) on complex64 states
+ * (lds_bytes only);
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad

@@ -680,6 +680,16 @@ extern "C" int ork_sv_1q(void *stream, int prec, void *psi, int n, int q, int op
     return prec == ORK_F32 ? sv_1q<float2>(st, psi, n, q, op, k) : sv_1q<double2>(st, psi, n, q, op, k);
 }
 
+// residency cap of the permutation / phase kernel (cx, cz, swap, ...) on complex64 states
+constexpr int kMonoLds32 = 26624;
+static int g_mono_lds32 = kMonoLds32;
+
+extern "C" int ork_sv_set_mono_lds32(int lds) {
+    if (lds < 0 || lds > 160 * 1024) return (int)hipErrorInvalidValue;
+    g_mono_lds32 = lds;
+    return 0;
+}
+
 extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_mono_t *mono) {
     MonoArgs m;
     m.k = mono->k;
@@ -705,15 +715,15 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     if (m.nt == 0) return 0;
     const u64 nbase = ((u64)1 << n) >> m.k;
     hipStream_t st = (hipStream_t)stream;
-    const int lds = 53248;   // same residency cap as the pair kernel
+    const int lds = prec == ORK_F32 ? g_mono_lds32 : 53248;   // residency cap (complex128: as the pair kernel)
     // streaming accesses when no gate qubit lies inside a 128 B line (as for the pair kernel)
     const u64 elem = prec == ORK_F32 ? 8 : 16;
     const bool nt = !(g_nt_off & 2) && (elem << m.pos[0]) >= 128 && (elem << n) >= g_nt_min_bytes;
     if (prec == ORK_F32) {
         const u64 per = (u64)SV_THREADS * SV_UNROLL * 2;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
-        if (nt) sv_mono_kernel<float2, SV_UNROLL * 2, 1><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
-        else sv_mono_kernel<float2, SV_UNROLL * 2, 0><<<grid, SV_THREADS, lds / 2, st>>>((float2 *)psi, nbase, m);
+        if (nt) sv_mono_kernel<float2, SV_UNROLL * 2, 1><<<grid, SV_THREADS, lds, st>>>((float2 *)psi, nbase, m);
+        else sv_mono_kernel<float2, SV_UNROLL * 2, 0><<<grid, SV_THREADS, lds, st>>>((float2 *)psi, nbase, m);
     } else {
         const u64 per = (u64)SV_THREADS * SV_UNROLL;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
@@ -817,6 +827,7 @@ extern "C" void ork_sv_reset_tuning(void) {
     g_layer_variant = kLayerVariant;
     g_red[0] = RedShape{kRedLds, kRedUnroll, kRedShift};
     g_red[1] = RedShape{kRedLds32, kRedUnroll32, kRedShift32};
+    g_mono_lds32 = kMonoLds32;
     g_exp_u[0] = kExpU;
     g_exp_u[1] = kExpU32;
     g_exp_lds[0] = kExpLds;
