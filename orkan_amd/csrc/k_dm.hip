@@ -1023,6 +1023,15 @@ extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
 extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, void *rho, int n, int opk,
                             const ork_mono_t *m, const double *mat) {
     const int k = m->k;
+    // only the two density-matrix layouts, with distinct gate qubits inside the state
+    if ((layout != ORK_PACKED && layout != ORK_TILED) || k < 1 || k > 3 || n < 1 || n > 31 || log_tile < 1 ||
+        log_tile > LW_MAX)
+        return (int)hipErrorInvalidValue;
+    for (int j = 0; j < k; ++j) {
+        if (m->qs[j] < 0 || m->qs[j] >= n) return (int)hipErrorInvalidValue;
+        for (int i = 0; i < j; ++i)
+            if (m->qs[i] == m->qs[j]) return (int)hipErrorInvalidValue;
+    }
     const DmPlan p = make_plan(layout, log_tile, n, m->qs, k);
     DmOp op;
     op.k = k;

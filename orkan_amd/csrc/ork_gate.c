@@ -224,13 +224,30 @@ void ccx_pure(state_t *s, qubit_t a, qubit_t b, qubit_t c) { g_ccx(s, a, b, c); 
 void ccx_packed(state_t *s, qubit_t a, qubit_t b, qubit_t c) { g_ccx(s, a, b, c); }
 void ccx_tiled(state_t *s, qubit_t a, qubit_t b, qubit_t c) { g_ccx(s, a, b, c); }
 
+/* single_from_mat / two_from_mat are the reference's internal tiled backends
+ * (src/gate/tiled/gate_tiled_1q.c:3, src/gate/tiled/gate_tiled_2q.c:3), which validate nothing; here
+ * they also serve the packed layout, and a bad argument stops the program with a message like every
+ * other gate instead of reaching the device */
+#define MV_MIXED(name, st)                                                                        \
+    ORK_VALIDATE("gate", (st) && (st)->data, name ": null state or data pointer");                \
+    ORK_VALIDATE("gate", ork_type_ok(st) && ork_layout(st) != PURE,                               \
+                 name ": state must be MIXED_PACKED or MIXED_TILED")
+
 void single_from_mat(state_t *s, qubit_t target, const cplx_t *mat) {
+    MV_MIXED("single_from_mat", s);
+    ORK_VALIDATE("gate", target < s->qubits, "single_from_mat: target qubit out of range");
+    ORK_VALIDATE("gate", mat, "single_from_mat: null matrix pointer");
     double S[32];
     sop_from_mat(S, mat);
     dm_sop(s, target, S);
 }
 
 void two_from_mat(state_t *s, qubit_t q1, qubit_t q2, const cplx_t *mat) {
+    MV_MIXED("two_from_mat", s);
+    ORK_VALIDATE("gate", q1 < s->qubits, "two_from_mat: q1 qubit out of range");
+    ORK_VALIDATE("gate", q2 < s->qubits, "two_from_mat: q2 qubit out of range");
+    ORK_VALIDATE("gate", q1 != q2, "two_from_mat: q1 and q2 must differ");
+    ORK_VALIDATE("gate", mat, "two_from_mat: null matrix pointer");
     const qubit_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
     ork_mono_t m;
     memset(&m, 0, sizeof(m));

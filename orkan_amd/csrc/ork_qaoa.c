@@ -36,6 +36,10 @@ struct orkan_qaoa {
     size_t states_bytes;
     /* launch configuration */
     int lt, threads, grid;
+    /* the graph is captured on this private stream, never on the library stream that other
+     * threads' gates and reductions use, and replayed on the library stream */
+    hipStream_t capture_stream;
+    int nevals_h;          /* host source of the nevals_d upload */
     hipGraph_t graph;
     hipGraphExec_t exec;
     int graph_ok;
@@ -78,8 +82,8 @@ static void drop_graph(orkan_qaoa_t *q) {
     q->graph_ok = 0;
 }
 
-int orkan_qaoa_set_config(orkan_qaoa_t *q, int lt, int threads, int blocks_per_cu) {
-    if (!q) return (int)hipErrorInvalidValue;
+/* caller holds ork_lock (the state scratch may be freed and the graph dropped) */
+static int set_config(orkan_qaoa_t *q, int lt, int threads, int blocks_per_cu) {
     if (2 * lt < q->nmax) return (int)hipErrorInvalidValue;     /* pass B needs n <= 2 LT */
     int occ = 0;
     int e = ork_qaoa_occupancy(lt, threads, &occ);
@@ -91,6 +95,14 @@ int orkan_qaoa_set_config(orkan_qaoa_t *q, int lt, int threads, int blocks_per_c
     q->threads = threads;
     q->grid = cus() * blocks_per_cu;
     return ensure_states(q);
+}
+
+int orkan_qaoa_set_config(orkan_qaoa_t *q, int lt, int threads, int blocks_per_cu) {
+    if (!q) return (int)hipErrorInvalidValue;
+    ork_lock();
+    const int e = set_config(q, lt, threads, blocks_per_cu);
+    ork_unlock();
+    return e;
 }
 
 orkan_qaoa_t *orkan_qaoa_create(int ngraphs, const unsigned char *nq, const double *const *diags, int max_evals,
@@ -108,6 +120,10 @@ orkan_qaoa_t *orkan_qaoa_create(int ngraphs, const unsigned char *nq, const doub
     q->order_h = calloc((size_t)max_evals, sizeof(int));
     q->angles_h = calloc((size_t)max_evals * max_p * 3, sizeof(double));
     if (!q->graphs_h || !q->evals_h || !q->order_h || !q->angles_h) goto fail;
+    if (hipStreamCreateWithFlags(&q->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        q->capture_stream = NULL;
+        goto fail;
+    }
     uint64_t total = 0;
     for (int g = 0; g < ngraphs; ++g) {
         if (nq[g] < 1 || nq[g] > 26) goto fail;
@@ -137,7 +153,10 @@ orkan_qaoa_t *orkan_qaoa_create(int ngraphs, const unsigned char *nq, const doub
          * largest state needs it */
         int lt = 12;
         while (2 * lt < q->nmax) ++lt;
-        if (orkan_qaoa_set_config(q, lt, lt >= 13 ? 1024 : 512, 0)) goto fail;
+        ork_lock();
+        const int e = set_config(q, lt, lt >= 13 ? 1024 : 512, 0);
+        ork_unlock();
+        if (e) goto fail;
     }
     return q;
 fail:
@@ -157,11 +176,12 @@ void orkan_qaoa_destroy(orkan_qaoa_t *q) {
     free(q->evals_h);
     free(q->order_h);
     free(q->angles_h);
+    if (q->capture_stream) (void)hipStreamDestroy(q->capture_stream);
     free(q);
 }
 
 static int capture(orkan_qaoa_t *q) {
-    hipStream_t st = (hipStream_t)ork_dev()->stream;
+    hipStream_t st = q->capture_stream;
     drop_graph(q);
     int e = (int)hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e) return e;
@@ -219,13 +239,13 @@ static int stage(orkan_qaoa_t *q, int nevals, const int *graph, const int *p, co
         e |= (int)hipMemcpyAsync(q->angles_d, q->angles_h, (size_t)nevals * q->max_p * 3 * sizeof(double),
                                  hipMemcpyHostToDevice, st);
     }
-    static int nv;
-    nv = nevals;
-    e |= (int)hipMemcpyAsync(q->nevals_d, &nv, sizeof(int), hipMemcpyHostToDevice, st);
+    q->nevals_h = nevals;
+    e |= (int)hipMemcpyAsync(q->nevals_d, &q->nevals_h, sizeof(int), hipMemcpyHostToDevice, st);
     e |= (int)hipStreamSynchronize(st);
     return e;
 }
 
+/* caller holds ork_lock */
 int orkan_qaoa_launch_staged(orkan_qaoa_t *q) {
     if (!q->graph_ok) {
         int e = capture(q);
@@ -265,17 +285,26 @@ int orkan_qaoa_autotune(orkan_qaoa_t *q, int nevals, const int *graph, const int
     if (!q) return (int)hipErrorInvalidValue;
     if (reps < 1) reps = 1;
     if (reps > 15) reps = 15;
-    ork_lock();
-    int e = stage(q, nevals, graph, p, gamma, beta);
-    ork_unlock();
-    if (e) return e;
     hipStream_t st = (hipStream_t)ork_dev()->stream;
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return (int)hipErrorOutOfMemory;
+    if (hipEventCreate(&a) != hipSuccess) return (int)hipErrorOutOfMemory;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return (int)hipErrorOutOfMemory;
+    }
+    /* the whole search holds the library lock: staging, every reconfiguration and launch */
+    ork_lock();
+    int e = stage(q, nevals, graph, p, gamma, beta);
+    if (e) {
+        ork_unlock();
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return e;
+    }
     double best = 1e30;
     int best_i = -1;
     for (size_t i = 0; i < sizeof(cand) / sizeof(cand[0]); ++i) {
-        if (orkan_qaoa_set_config(q, cand[i][0], cand[i][1], 0)) continue;
+        if (set_config(q, cand[i][0], cand[i][1], 0)) continue;
         if (orkan_qaoa_launch_staged(q)) continue;
         float t[15];
         int ok = 1;
@@ -298,9 +327,14 @@ int orkan_qaoa_autotune(orkan_qaoa_t *q, int nevals, const int *graph, const int
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (best_i < 0) return (int)hipErrorInvalidConfiguration;
+    if (best_i < 0) {
+        ork_unlock();
+        return (int)hipErrorInvalidConfiguration;
+    }
     if (best_ms) *best_ms = best;
-    return orkan_qaoa_set_config(q, cand[best_i][0], cand[best_i][1], 0);
+    e = set_config(q, cand[best_i][0], cand[best_i][1], 0);
+    ork_unlock();
+    return e;
 }
 
 int orkan_maxcut_diag_dev(int n, int nedges, const int *ei, const int *ej, const double *w, double *out_dev) {

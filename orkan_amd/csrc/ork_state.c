@@ -404,6 +404,24 @@ qs_error_t orkan_state_save(const state_t *state, const char *path) {
     return rc;
 }
 
+/* stored bytes of a (layout, n, element size) claimed by a file header, in 128-bit arithmetic so that
+ * no header can wrap them; 0 when the layout is unknown or the bytes exceed ORK_FILE_MAX_BYTES */
+#define ORK_FILE_MAX_BYTES ((unsigned __int128)1 << 56)
+static size_t file_bytes(int layout, unsigned n, uint64_t length, uint32_t elem) {
+    if (n > 63) return 0;
+    const unsigned __int128 dim = (unsigned __int128)1 << n;
+    unsigned __int128 len;
+    if (layout == PURE) len = dim;
+    else if (layout == MIXED_PACKED) len = n == 0 ? 1 : (dim >> 1) * (dim + 1);
+    else if (layout == MIXED_TILED) {
+        const unsigned __int128 nt = (dim + TILE_DIM - 1) >> LOG_TILE_DIM;
+        len = (nt * (nt + 1) >> 1) * (unsigned __int128)TILE_SIZE;
+    } else return 0;
+    const unsigned __int128 bytes = len * elem;
+    if (len != length || bytes == 0 || bytes > ORK_FILE_MAX_BYTES) return 0;
+    return (size_t)bytes;
+}
+
 qs_error_t orkan_state_load(state_t *state, const char *path) {
     if (!state || !path) return QS_ERR_NULL;
     FILE *f = fopen(path, "rb");
@@ -419,10 +437,11 @@ qs_error_t orkan_state_load(state_t *state, const char *path) {
         const int layout = (int)h.type & 0xff;
         if (memcmp(h.magic, ORK_FILE_MAGIC, 8) != 0 || h.version != ORK_FILE_VERSION || h.byte_order != 0x01020304u ||
             h.qubits > 63 || !ork_type_ok(&probe) || h.elem_bytes != ork_elem_bytes(&probe) ||
-            h.length != ork_len(layout, (qubit_t)h.qubits) || (layout == MIXED_TILED && h.log_tile != LOG_TILE_DIM))
+            !file_bytes(layout, h.qubits, h.length, h.elem_bytes) ||
+            (layout == MIXED_TILED && h.log_tile != LOG_TILE_DIM))
             rc = QS_ERR_FORMAT;
     }
-    const size_t total = rc == QS_OK ? (size_t)h.length * h.elem_bytes : 0;
+    const size_t total = rc == QS_OK ? file_bytes((int)h.type & 0xff, h.qubits, h.length, h.elem_bytes) : 0;
     void *dev = NULL;
     if (rc == QS_OK) {
         /* the payload must be exactly the header's length */

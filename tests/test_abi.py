@@ -115,6 +115,20 @@ CASES = [
      "orkan: meas: matel_diag: qubit count mismatch"),
     (0, 2, "lib.exp_diag(C.byref(st), None, 1.0)", "orkan: circ: exp_diag: null diagonal pointer"),
     (7, 2, "lib.h(C.byref(st), 0)", "orkan: gate: h: unknown state type"),
+    # the internal tiled backends the reference exports; here validated before any device call
+    (0, 3, "lib.single_from_mat(C.byref(st), 0, C.addressof(obs))",
+     "orkan: gate: single_from_mat: state must be MIXED_PACKED or MIXED_TILED"),
+    (2, 3, "lib.single_from_mat(C.byref(st), 3, C.addressof(obs))",
+     "orkan: gate: single_from_mat: target qubit out of range"),
+    (1, 3, "lib.single_from_mat(C.byref(st), 0, None)", "orkan: gate: single_from_mat: null matrix pointer"),
+    (2, 3, "st.data = None; lib.single_from_mat(C.byref(st), 0, C.addressof(obs))",
+     "orkan: gate: single_from_mat: null state or data pointer"),
+    (0, 3, "lib.two_from_mat(C.byref(st), 0, 1, C.addressof(obs))",
+     "orkan: gate: two_from_mat: state must be MIXED_PACKED or MIXED_TILED"),
+    (2, 3, "lib.two_from_mat(C.byref(st), 0, 3, C.addressof(obs))", "orkan: gate: two_from_mat: q2 qubit out of range"),
+    (1, 3, "lib.two_from_mat(C.byref(st), 4, 0, C.addressof(obs))", "orkan: gate: two_from_mat: q1 qubit out of range"),
+    (2, 3, "lib.two_from_mat(C.byref(st), 1, 1, C.addressof(obs))", "orkan: gate: two_from_mat: q1 and q2 must differ"),
+    (1, 3, "lib.two_from_mat(C.byref(st), 0, 1, None)", "orkan: gate: two_from_mat: null matrix pointer"),
 ]
 
 

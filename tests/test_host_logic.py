@@ -114,6 +114,12 @@ def test_state_file_errors_without_device(tmp_path):
         "tile.bin": header(typ=2, qubits=5, lt=3, length=state_len(2, 5, 5)) + bytes(state_len(2, 5, 5) * 16),
         "truncated.bin": header() + bytes(7 * 16),
         "long.bin": header() + bytes(9 * 16),
+        # lengths whose byte count wraps 64 bits (2^60 x 16 = 2^64 -> 0), or whose packed length does
+        # (D (D + 1) / 2 at D = 2^62): rejected before any allocation
+        "wrap_pure.bin": header(qubits=60, length=1 << 60) + bytes(0),
+        "wrap_packed.bin": header(typ=1, qubits=62, length=((1 << 61) * ((1 << 62) + 1)) % (1 << 64)),
+        "wrap_fp32.bin": header(typ=0x100, qubits=61, length=1 << 61, eb=8),
+        "huge.bin": header(qubits=57, length=1 << 57) + bytes(16),
     }
     for name, blob in cases.items():
         p = tmp_path / name

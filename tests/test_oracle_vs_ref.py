@@ -80,7 +80,7 @@ def ref_swap_defect(kind, name, qs, log_tile):
 @pytest.mark.parametrize("kind", KINDS)
 def test_2q_3q_gates(log_tile, kind):
     o, r = _pair(log_tile)
-    defects = 0
+    defects = candidates = 0
     for n in _ns(kind, log_tile):
         for a, b in itertools.permutations(range(n), 2):
             for name in ["cx", "cy", "cz", "swap_gate"]:
@@ -92,6 +92,7 @@ def test_2q_3q_gates(log_tile, kind):
                     want = G @ rho @ G.conj().T
                     assert np.abs(from_layout(o.read(so), kind, n, log_tile) - want).max() <= TOL
                     defects += np.abs(from_layout(r.read(sr), kind, n, log_tile) - want).max() > TOL
+                    candidates += 1
                 else:
                     _check(o, r, so, sr)
                 o.free(so); r.free(sr)
@@ -100,6 +101,10 @@ def test_2q_3q_gates(log_tile, kind):
             o.gate(so, "ccx", a, b, c); r.gate(sr, "ccx", a, b, c)
             _check(o, r, so, sr)
             o.free(so); r.free(sr)
+    # the documented divergence stays pinned: the reference deviates from U rho U^dagger on every
+    # such case (a fixed reference would fail here, and the exception above would then be removed)
+    assert defects == candidates
+    assert candidates == (0 if kind != MIXED_TILED else 2)
 
 
 @pytest.mark.parametrize("log_tile", [3, 5])
@@ -200,20 +205,32 @@ def test_generic_matrices_tiled(log_tile):
 
 
 def test_kraus_to_superop():
+    """the oracle's, the compiled reference's and the shipped library's kraus_to_superop
+    (orkan_amd/csrc/ork_channel.c; host code, no device call) against sum_a kron(K_a, conj K_a)
+    (reference src/channel/channel.c:21-50)"""
     import ctypes as C
+    import orkan_amd
     from orkan_amd import _ffi
+
+    class Shipped:
+        api = orkan_amd.load(5)
+        libc = C.CDLL(None)
+
     o, r = _pair(5)
     rng = np.random.default_rng(3)
-    for nq in (1, 2):
+    for nq in (1, 2, 3):
         N = 1 << nq
         ks = rng.standard_normal((3, N, N)) + 1j * rng.standard_normal((3, N, N))
         flat = np.ascontiguousarray(ks.reshape(-1))
         kr = _ffi.KrausT(nq, 3, flat.ctypes.data)
         outs = []
-        for eng in (o, r):
+        for eng in (o, r, Shipped):
             so = eng.api.kraus_to_superop(C.byref(kr))
+            assert so.n_qubits == nq
             arr = np.ctypeslib.as_array(C.cast(so.data, C.POINTER(C.c_double)), shape=(2 * N ** 4,)).copy()
             outs.append(arr.view(np.complex128))
             eng.libc.free(C.c_void_p(so.data))
-        assert np.abs(outs[0] - outs[1]).max() <= TOL
-        assert np.abs(outs[0] - sum(np.kron(K, K.conj()) for K in ks).reshape(-1)).max() <= TOL
+        want = sum(np.kron(K, K.conj()) for K in ks).reshape(-1)
+        for got in outs:
+            assert np.abs(got - want).max() <= TOL
+        assert np.abs(outs[2] - outs[1]).max() <= TOL
