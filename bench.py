@@ -66,21 +66,23 @@ class Events:
 
 
 def traffic_from_profiles(kernel_key):
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (profiles/traffic.json, written from the committed counter summaries)."""
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE in separate counter runs
+    (not in this timed run): profiles/traffic.json, written from the committed counter summaries.
+    Returns (bytes, source) with the summary file the figure comes from."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
-        return None
+        return None, None
     try:
-        return json.load(open(p)).get(kernel_key, {}).get("bytes_per_launch")
+        e = json.load(open(p)).get(kernel_key, {})
     except (OSError, ValueError):
-        return None
+        return None, None
+    return e.get("bytes_per_launch"), e.get("source", "profiles/traffic.json")
 
 
 # -------------------------------------------------------------------------------------------- sv28
 def run_sv28(args, lib):
     from orkan_amd import ORKAN_FP32, StateT
-    from orkan_amd import workloads as wl
+    from benchmarks import workloads as wl
 
     n = wl.SV_QUBITS
     steps = args.steps if args.steps is not None else wl.SV_LAYERS
@@ -179,6 +181,7 @@ def run_sv28(args, lib):
     fused = fused_layer(lib, ps, n, elem, layers[args.warmup] if steps > 0 else layers[0], rot_bytes, rot_avg)
 
     value = total_bytes / (total_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles("sv_pair_kernel_rot_28q" + ("_fp32" if args.fp32 else ""))
     lib.orkan_dev_free(obs_dev)
     lib.state_free(ps)
     result = {
@@ -200,7 +203,7 @@ def run_sv28(args, lib):
                    "storage": "complex64" if args.fp32 else "complex128", "seed": hex(wl.SEED_BASE + 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": traffic_from_profiles("sv_pair_kernel_rot_28q" + ("_fp32" if args.fp32 else "")),
+                     "traffic": traffic[0], "traffic_source": traffic[1],
                      "kernel": "sv_pair_kernel (rx/ry/rz)", "bytes_per_launch": rot_bytes,
                      "avg_launch_ms": round(rot_avg, 4), "launches": int(rot_ms.size)},
         "per_gate_GBs": per_gate,
@@ -260,7 +263,7 @@ def sv_fp32_block(lib, layers, n, nlayers=2):
     complex128 states from |+>; rotation-kernel GB/s (8 B per amplitude) and the
     max amplitude error against complex128."""
     from orkan_amd import ORKAN_FP32, StateT
-    from orkan_amd import workloads as wl
+    from benchmarks import workloads as wl
     s64, s32 = StateT(0, None, 0), StateT(0 | ORKAN_FP32, None, 0)
     lib.state_plus(C.byref(s64), n)
     lib.state_plus(C.byref(s32), n)
@@ -298,48 +301,56 @@ def sv_fp32_block(lib, layers, n, nlayers=2):
             "max_abs_amplitude_err_vs_fp64": err, "norm_drift": abs(norm - 1.0)}
 
 
-def cpu_baseline_sv(layer, n, elem=16, budget_s=20.0):
-    """The reference library (oracle/_ref, compiled from its sources) on this
-    host at the same n, OpenMP over all cores: the gates of one brick layer in
-    order, stopping after `budget_s` seconds of CPU work."""
-    os.environ.setdefault("OMP_PROC_BIND", "spread")
-    os.environ.setdefault("OMP_PLACES", "cores")
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    os.environ["OMP_NUM_THREADS"] = str(cores)
-    from orkan_amd import _ffi
-    kind = "reference"
-    path = os.path.join(ROOT, "oracle", "_ref", "liborkan_ref_t5.so")
-    if not os.path.exists(path):
-        kind = "port"
-        path = os.path.join(ROOT, "oracle", "lib", "liborkan_oracle_t5.so")
-    if not os.path.exists(path):
+def cpu_baseline_sv(layer, n, elem=16):
+    """The reference library (oracle/_ref, compiled from its sources; the restatement where it is
+    absent) on this host at the same n, OpenMP over the cores this process may use.  Per gate type
+    the reference's two-level method (benchmarks/cpu.py): one sweep of the layer's positions per
+    timed block (rx, ry, rz: every qubit; cx: the layer's pairs), 3 warm-up sweeps, 5 samples,
+    median per application; GB/s weighs the medians by one layer's gate counts.  One whole layer in
+    order is timed beside it."""
+    from benchmarks import cpu as cpu_mod
+    from benchmarks import workloads as wl
+    threads = cpu_mod.setup_threads()
+    kind, path = cpu_mod.library(5)
+    if path is None:
         return None
+    from orkan_amd import _ffi
     ref = _ffi.bind(path)
     st = _ffi.StateT(0, None, 0)
     ref.state_plus(C.byref(st), n)
     ps = C.byref(st)
-    # one untimed gate to fault the pages in
-    ref.rx(ps, 0, 0.1)
+    ang = {}
+    for op in layer:
+        if op[0] != "cx":
+            ang.setdefault(op[0], {})[op[1]] = op[2]
+    pairs = [(op[1], op[2]) for op in layer if op[0] == "cx"]
+    sweeps = {g: (lambda g=g: [getattr(ref, g)(ps, q, ang[g][q]) for q in range(n)]) for g in ("rx", "ry", "rz")}
+    sweeps["cx"] = lambda: [ref.cx(ps, a, b) for a, b in pairs]
+    positions = {"rx": n, "ry": n, "rz": n, "cx": len(pairs)}
+    per = {}
+    for g, fn in sweeps.items():
+        r = cpu_mod.two_level(fn, warmup=3, samples=5)
+        for k in ("median_s", "min_s", "max_s"):
+            r[k] /= positions[g]
+        per[g] = r
+        log(f"cpu baseline: {g} {r['median_s'] * 1e3:.2f} ms per gate (median of 5 sweeps)")
+    gbs, t_layer = cpu_mod.weighted(per, positions, lambda g: wl.sv_gate_bytes(g, n, elem))
     t0 = time.perf_counter()
-    nbytes = 0
-    done = 0
-    from orkan_amd import workloads as wl
     for op in layer:
         if op[0] == "cx":
             ref.cx(ps, op[1], op[2])
         else:
             getattr(ref, op[0])(ps, op[1], op[2])
-        nbytes += wl.sv_gate_bytes(op[0], n, elem)
-        done += 1
-        if done % 8 == 0:
-            log(f"cpu baseline: {done} gates, {time.perf_counter() - t0:.1f} s")
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    wall = time.perf_counter() - t0
     ref.state_free(ps)
-    return {"value": round(nbytes / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": kind,
-            "sample": f"first {done} of the {len(layer)} gates of one brick layer of the same {n}-qubit "
-                      f"circuit, complex128, {dt:.2f} s", "seconds": round(dt, 3)}
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": threads, "kind": kind,
+            "sample": f"reference two-level method per gate type (rx, ry, rz over all {n} qubits, cx over the "
+                      f"layer's {len(pairs)} pairs), 5 samples of one sweep after 3 warm-up sweeps, medians weighted "
+                      f"by one brick layer's gate counts; {n}-qubit complex128 state",
+            "median_ms_per_gate": {g: round(v["median_s"] * 1e3, 3) for g, v in per.items()},
+            "GB/s_per_gate": {g: round(wl.sv_gate_bytes(g, n, elem) / v["median_s"] / 1e9, 2) for g, v in per.items()},
+            "layer_ms_weighted": round(t_layer * 1e3, 2), "layer_wall_ms": round(wall * 1e3, 2),
+            "circuit_s_projected": round(40 * wall, 1), "host": cpu_mod.host_info()}
 
 
 def main():
@@ -356,22 +367,24 @@ def main():
         if not args.no_cpu:
             result["cpu_baseline"] = cpu_baseline_sv(sample_layer, n)
         if not args.quick:
-            from orkan_amd import workloads as wl
-            from orkan_amd import bench_dm, bench_qaoa
+            from benchmarks import workloads as wl
+            from benchmarks import dm as bench_dm, qaoa as bench_qaoa, screening
             log("fp32 statevector block")
             lay, _ = wl.brick_circuit(n, 2)
             result["fp32_storage"] = sv_fp32_block(lib, lay, n)
             dm_args = argparse.Namespace(steps=2, warmup=1)
-            result["dm14"] = bench_dm.run(dm_args, lib, Events)
+            result["dm14"] = bench_dm.run(dm_args, lib, Events, cpu=not args.no_cpu)
             result["qaoa"] = bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)
+            log("surrogate block (config 5: 10^6 candidates per held-out graph, 4 graphs)")
+            result["surrogate"] = screening.run(lib, test_graphs=4)
     elif args.workload == "dm14":
-        from orkan_amd import bench_dm
-        result = {"metric": METRIC, "dm14": bench_dm.run(args, lib, Events)}
+        from benchmarks import dm as bench_dm
+        result = {"metric": METRIC, "dm14": bench_dm.run(args, lib, Events, cpu=not args.no_cpu)}
     elif args.workload == "qaoa":
-        from orkan_amd import bench_qaoa
+        from benchmarks import qaoa as bench_qaoa
         result = {"metric": METRIC, "qaoa": bench_qaoa.run(lib, ROOT, cpu_budget_s=0.0 if args.no_cpu else 12.0)}
     else:
-        from orkan_amd import bench_surrogate
+        from benchmarks import screening as bench_surrogate
         result = {"metric": METRIC, "surrogate": bench_surrogate.run(lib)}
     print(json.dumps(result))
 

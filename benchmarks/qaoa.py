@@ -14,7 +14,8 @@ import time
 
 import numpy as np
 
-from . import batch, workloads as wl
+from orkan_amd import batch
+from . import workloads as wl
 
 
 def log(msg):
@@ -75,7 +76,7 @@ def run(lib, root, reps=5, cpu_budget_s=15.0, count=4096):
     if not os.path.exists(ref_path):
         ref_path, kind = os.path.join(root, "oracle", "lib", "liborkan_oracle_t5.so"), "port"
     if os.path.exists(ref_path) and cpu_budget_s > 0:
-        from . import _ffi
+        from orkan_amd import _ffi
         ref = _ffi.bind(ref_path)
         t0 = time.perf_counter()
         done, worst = 0, 0.0

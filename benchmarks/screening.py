@@ -25,7 +25,8 @@ import time
 
 import numpy as np
 
-from . import batch, surrogate, workloads as wl
+from orkan_amd import batch, surrogate
+from . import workloads as wl
 
 N, P = 16, 4
 

@@ -7,6 +7,8 @@
 #ifndef SAMPLE_H
 #define SAMPLE_H
 
+#include <stddef.h>
+
 #include "state.h"
 
 #ifdef __cplusplus

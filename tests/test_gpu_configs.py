@@ -20,7 +20,7 @@ import pytest
 import engines
 import orkan_amd
 from linalg_np import MIXED_PACKED, MIXED_TILED, PURE, from_layout, tiled_diag_mirror_error, to_layout
-from orkan_amd import workloads as wl
+from benchmarks import workloads as wl
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12

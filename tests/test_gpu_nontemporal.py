@@ -16,7 +16,7 @@ import pytest
 import engines
 import orkan_amd
 from linalg_np import MIXED_PACKED, MIXED_TILED, PURE, to_layout
-from orkan_amd import workloads as wl
+from benchmarks import workloads as wl
 
 pytestmark = pytest.mark.gpu
 NT_OFF_HOOK, RESET_HOOK = 13, 14

@@ -75,3 +75,46 @@ def test_autotuner_picks_a_valid_config():
     assert ms > 0 and cfg["lt"] in (11, 12, 13)
     assert np.abs(b.eval(*args) - ref).max() <= 1e-12
     b.close()
+
+
+def test_largest_states_n19_n20():
+    """States of 19 and 20 qubits (the top of the 10..20 range of the 4096-instance workload) stream
+    through HBM in p + 1 LDS-tiled sweeps; each against its per-instance oracle run."""
+    lib = orkan_amd.load(5)
+    orc = engines.oracle(5)
+    rng = np.random.default_rng(1920)
+    inst = batch.maxcut_instances(6, rng, n_lo=19, n_hi=20)
+    diags = [batch.maxcut_diag(i["n"], i["edges"], i["w"]) for i in inst]
+    b = batch.QaoaBatch(lib, [i["n"] for i in inst], diags, max_evals=8, max_p=8)
+    got = b.eval(list(range(len(inst))), [i["p"] for i in inst], [i["gamma"] for i in inst],
+                 [i["beta"] for i in inst])
+    for k, i in enumerate(inst):
+        print(f"n {i['n']} p {i['p']}", flush=True)
+        want = oracle_eval(orc, i["n"], diags[k], i["p"], i["gamma"], i["beta"])
+        assert abs(got[k] - want) <= 1e-12, (k, i["n"], i["p"], got[k], want)
+    b.close()
+
+
+def test_full_4096_batch_sample():
+    """The 4096-instance workload itself (cost diagonals built on the device), autotuned, one
+    launch; a seeded sample of its evaluations, always including the largest states, against
+    per-instance oracle runs at 1e-12."""
+    from benchmarks import qaoa as bq
+    lib = orkan_amd.load(5)
+    orc = engines.oracle(5)
+    inst = bq.make(4096)
+    dptrs = bq.device_diags(lib, inst)
+    b = batch.QaoaBatch(lib, [i["n"] for i in inst], dptrs, max_evals=len(inst), max_p=8)
+    for p in dptrs:
+        lib.orkan_dev_free(p)
+    args = (list(range(len(inst))), [i["p"] for i in inst], [i["gamma"] for i in inst], [i["beta"] for i in inst])
+    _, cfg = b.autotune(*args, reps=1)
+    print(f"autotuned {cfg}", flush=True)
+    got = b.eval(*args)
+    b.close()
+    order = np.argsort([-(i["n"] * 8 + i["p"]) for i in inst], kind="stable")
+    sample = list(order[:4]) + list(np.random.default_rng(4096).choice(len(inst), 20, replace=False))
+    for k in sample:
+        i = inst[k]
+        want = oracle_eval(orc, i["n"], batch.maxcut_diag(i["n"], i["edges"], i["w"]), i["p"], i["gamma"], i["beta"])
+        assert abs(got[k] - want) <= 1e-12, (k, i["n"], i["p"], got[k], want)

@@ -10,7 +10,7 @@ import pytest
 import engines
 import orkan_amd
 from linalg_np import PURE
-from orkan_amd import workloads as wl
+from benchmarks import workloads as wl
 
 pytestmark = pytest.mark.gpu
 RED_HOOK, RESET_HOOK = 10, 14

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import orkan_amd
-from orkan_amd import bench_surrogate
+from benchmarks import screening
 
 pytestmark = pytest.mark.gpu
 
@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 def test_surrogate_gain_over_plain_search():
     lib = orkan_amd.load(5)
     topk, ncand = 64, 65536
-    r = bench_surrogate.run(lib, train_graphs=512, per_graph=2048, test_graphs=4, ncand=ncand, topk=topk,
-                            epochs=16, batch_size=4096, plain_batch=65536)
+    r = screening.run(lib, train_graphs=512, per_graph=2048, test_graphs=4, ncand=ncand, topk=topk,
+                      epochs=16, batch_size=4096, plain_batch=65536)
     print({k: v for k, v in r.items() if k != "search"})
     for row in r["search"]:
         print(row)

@@ -3,8 +3,9 @@ surrogate features, MaxCut diagonals, layout helpers."""
 import numpy as np
 
 from linalg_np import MIXED_PACKED, MIXED_TILED, from_layout, random_rho, state_len, to_layout
-from orkan_amd import batch, surrogate, workloads as wl
-from orkan_amd.bench_dm import build_rho_layout
+from benchmarks import workloads as wl
+from orkan_amd import batch, surrogate
+from benchmarks.dm import build_rho_layout
 
 
 def test_brick_layer_structure():
