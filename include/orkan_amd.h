@@ -79,6 +79,26 @@ int orkan_copy_to_host(void *dst_host, const void *src_dev, size_t bytes);
 int orkan_copy_to_device(void *dst_dev, const void *src_host, size_t bytes);
 
 /* ------------------------------------------------------------------------------------------------
+ * Allocation-free exp_diag.  No counterpart in the reference (whose exp_diag, include/circ.h:41,
+ * is declared in circ.h here unchanged): an entry point for callers that cannot allocate device
+ * memory -- code under HIP graph capture of the library stream (orkan_stream), tight serving
+ * loops, harnesses that forbid hipMalloc.  Same result as exp_diag(state, diag_dev, t) (the same
+ * kernels on the library stream, enqueued and not waited for), but it allocates and frees nothing:
+ * the per-index phase scratch of a density matrix is a fixed device array of the library,
+ * 2^ORKAN_NOALLOC_MAX_QUBITS complex128 (2 MiB, held on every GPU the library runs on until the
+ * process ends), and diag_dev must already be device memory (orkan_dev_alloc), never staged.
+ * Returns QS_OK (also for t == 0, which leaves the state untouched); QS_ERR_NULL; QS_ERR_TYPE;
+ * QS_ERR_QUBIT for a density matrix of more than ORKAN_NOALLOC_MAX_QUBITS qubits (PURE states use
+ * no scratch and have no such limit); QS_ERR_PARAM (with a stderr line) when diag_dev is not
+ * device memory, when no state has been created on the current device yet, or when the launch
+ * fails.  Every call on one GPU uses the same scratch array, in stream order on the library
+ * stream: calls that overlap on one GPU -- graphs captured from it and replayed concurrently on
+ * other streams -- race.
+ * ---------------------------------------------------------------------------------------------- */
+#define ORKAN_NOALLOC_MAX_QUBITS 17
+int orkan_exp_diag_noalloc(state_t *state, const double *diag_dev, double t);
+
+/* ------------------------------------------------------------------------------------------------
  * Stream and timing.  All kernels of the library run in order on one stream
  * per device; gates return as soon as their kernel is enqueued, and every call
  * that returns host data waits for it.

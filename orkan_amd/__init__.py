@@ -37,6 +37,7 @@ EXTRA = {
     "orkan_event_record": (C.c_int, [C.c_int]),
     "orkan_event_ms": (C.c_double, [C.c_int, C.c_int]),
     "orkan_set_launch_shape": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "orkan_exp_diag_noalloc": (C.c_int, [_ffi.PS, C.c_void_p, C.c_double]),
 }
 
 

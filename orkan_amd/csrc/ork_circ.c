@@ -53,3 +53,38 @@ void exp_diag(state_t *state, const double *diag, double t) {
     XV(ork_type_ok(state), "exp_diag: invalid state type");
     exp_any(state, diag, t);
 }
+
+/* Allocation-free exp_diag (include/orkan_amd.h): the same kernels as exp_diag on the library
+ * stream, with the density-matrix phase scratch in the fixed device array of k_noalloc.hip, the
+ * diagonal taken from device memory only (no staging), and nothing allocated or freed. */
+_Static_assert(ORK_NOALLOC_MAX_QUBITS == ORKAN_NOALLOC_MAX_QUBITS, "scratch size of the header and the kernels");
+
+int orkan_exp_diag_noalloc(state_t *state, const double *diag_dev, double t) {
+    if (!state || !state->data || !diag_dev) return QS_ERR_NULL;
+    if (!ork_type_ok(state)) return QS_ERR_TYPE;
+    if (ork_layout(state) != PURE && state->qubits > ORK_NOALLOC_MAX_QUBITS) {
+        fprintf(stderr,
+                "orkan: circ: orkan_exp_diag_noalloc: a density matrix of %d qubits exceeds the fixed phase scratch "
+                "(%d qubits, %llu B)\n",
+                (int)state->qubits, ORK_NOALLOC_MAX_QUBITS, (unsigned long long)16 << ORK_NOALLOC_MAX_QUBITS);
+        return QS_ERR_QUBIT;
+    }
+    if (!ork_is_dev_ptr(diag_dev)) {
+        fprintf(stderr, "orkan: circ: orkan_exp_diag_noalloc: the diagonal must be device memory\n");
+        return QS_ERR_PARAM;
+    }
+    if (t == 0.0) return QS_OK;
+    ork_dev_t *d = ork_dev_existing();
+    if (!d || d->noalloc_err) {
+        fprintf(stderr, "orkan: circ: orkan_exp_diag_noalloc: library runtime not set up on this device (%s)\n",
+                d ? hipGetErrorString((hipError_t)d->noalloc_err) : "no state created on it yet");
+        return QS_ERR_PARAM;
+    }
+    const int e = ork_exp_diag_noalloc(d->stream, ork_prec(state), ork_layout(state), LOG_TILE_DIM, state->data,
+                                       state->qubits, diag_dev, t);
+    if (e) {
+        fprintf(stderr, "orkan: circ: orkan_exp_diag_noalloc: %s\n", hipGetErrorString((hipError_t)e));
+        return QS_ERR_PARAM;
+    }
+    return QS_OK;
+}

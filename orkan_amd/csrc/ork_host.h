@@ -40,10 +40,15 @@ typedef struct {
     size_t stage_bytes;
     void *phase;           /* exp_diag per-index phases */
     size_t phase_bytes;
+    int noalloc_err;       /* result of ork_noalloc_setup() on this device */
 } ork_dev_t;
 
 /* runtime of the current device (created on first use) */
 ork_dev_t *ork_dev(void);
+/* runtime of the current device if it has been created, else NULL (creates nothing) */
+ork_dev_t *ork_dev_existing(void);
+/* does p address device memory? */
+int ork_is_dev_ptr(const void *p);
 void ork_lock(void);
 void ork_unlock(void);
 

@@ -82,6 +82,16 @@ int ork_dm_exp_diag(void *stream, int prec, int layout, int log_tile, void *rho_
 int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rho_dev, int n,
                 const double *obs_dev, double *partials_dev, double *out_host);
 
+/* Allocation-free exp_diag: the phase scratch of a density matrix is a fixed device array of
+ * 2^ORK_NOALLOC_MAX_QUBITS complex128 (k_noalloc.hip).  ork_noalloc_setup takes that array's
+ * address on the current device (once per device, at runtime creation); ork_exp_diag_noalloc
+ * launches exp_diag with it (hipErrorInvalidValue for a density matrix above the maximum,
+ * hipErrorNotInitialized before the set-up).  Calls that overlap on one GPU share the array. */
+#define ORK_NOALLOC_MAX_QUBITS 17
+int ork_noalloc_setup(void);
+int ork_exp_diag_noalloc(void *stream, int prec, int layout, int log_tile, void *data_dev, int n,
+                         const double *diag_dev, double t);
+
 /* launch shape of the density-matrix kernels: which 0 = tile-group kernel, 1 = phase kernel;
  * xcd: keep consecutive groups on one XCD; lds_bytes: LDS per workgroup (caps residency) */
 int ork_dm_set_shape(int which, int flags, int lds_bytes);

@@ -38,6 +38,9 @@ ork_dev_t *ork_dev(void) {
             ORK_HIP("hipStreamCreateWithFlags", hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             d->stream = (void *)st;
             ORK_HIP("hipMalloc(partials)", hipMalloc((void **)&d->partials, ork_partials_len() * sizeof(double)));
+            /* device address of the allocation-free exp_diag's scratch array on this device; a failure
+             * leaves it unset, and only orkan_exp_diag_noalloc reports it */
+            d->noalloc_err = ork_noalloc_setup();
             __atomic_store_n(&g_ready[dev], 1, __ATOMIC_RELEASE);
         }
         pthread_mutex_unlock(&g_init_mu);
@@ -45,7 +48,13 @@ ork_dev_t *ork_dev(void) {
     return &g_dev[dev];
 }
 
-static int is_device_ptr(const void *p) {
+ork_dev_t *ork_dev_existing(void) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ORK_MAX_DEV) return NULL;
+    return __atomic_load_n(&g_ready[dev], __ATOMIC_ACQUIRE) ? &g_dev[dev] : NULL;
+}
+
+int ork_is_dev_ptr(const void *p) {
     hipPointerAttribute_t at;
     memset(&at, 0, sizeof(at));
     if (hipPointerGetAttributes(&at, p) != hipSuccess) {
@@ -56,7 +65,7 @@ static int is_device_ptr(const void *p) {
 }
 
 const double *ork_dev_doubles(const double *src, size_t count) {
-    if (is_device_ptr(src)) return src;
+    if (ork_is_dev_ptr(src)) return src;
     ork_dev_t *d = ork_dev();
     const size_t bytes = count * sizeof(double);
     if (d->stage_bytes < bytes) {
