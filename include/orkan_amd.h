@@ -131,13 +131,22 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * (kernel 9 then applies to complex128 only), 19: statevector exp_diag (flags bits 0-1: log2 of the
  * amplitudes per thread, 0..2, bit 2: the shape of complex64 states, else of complex128 ones;
  * lds_bytes as below), 20: permutation / phase kernels (cx, cz, swap, ...) on complex64 states
- * (lds_bytes only);
+ * (lds_bytes only), 21: statevector permutations (cx, cy, swap, ccx, ...): flags bits 0-1 log2 of
+ * the lane-exchange kernel's amplitudes per thread (1..3), bit 2 whole-line writes, bit 3 lane-exchange
+ * kernel off, bit 4 lane-exchange kernel for every gate whose qubits are all below 6 (else only where
+ * one of them lies inside a 128 B line), bit 5 paired complex64 kernel (two amplitudes per 16 B access,
+ * no gate qubit at bit 0) off, bit 6 the amplitudes per thread and lds_bytes are those of complex64
+ * states, bit 7 lds_bytes is the paired kernel's cap instead; 22: density-matrix permutations whose qubits
+ * all lie at or above the tile (32 x 32 blocks for packed), which move whole tiles: flags bits 0 / 5 as
+ * below, bit 1 sends them to the tile-group kernel instead, bits 3-4 log2 of the workgroups that share a
+ * tile pair whose source needs no transpose (32 x 32 tiles, 0..2);
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad
  * or packed block (0..2), bit 3 of kernels 6, 15 (with bit 2) selects the lane-exchange kernel for
  * in-tile targets whose quad columns share 128 B lines, bit 5 hands each XCD runs of 32 consecutive
- * groups instead of the contiguous ranges of bit 0;
+ * groups instead of the contiguous ranges of bit 0, bit 6 (phase kernels 1, 4, 5) writes every 128 B
+ * line holding a changed entry in full where the changed entries do not fill lines;
  * lds_bytes caps workgroups per CU (0: no cap).  The settings are process-wide and unsynchronised:
  * change them while no other thread is calling into the library.  Returns 0, or hipErrorInvalidValue
  * for a kernel id or value outside the ranges above. */

@@ -80,6 +80,13 @@ __device__ __forceinline__ double shfl_xor_f64(double v, int m) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ __forceinline__ cd shfl_xor_cd(cd v, int m) { return cmk(shfl_xor_f64(v.x, m), shfl_xor_f64(v.y, m)); }
+// the value of lane `src` (any lane of the wavefront; every lane takes part)
+__device__ __forceinline__ double shfl_f64(double v, int src) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __shfl((int)(unsigned)b, src, 64), hi = __shfl((int)(unsigned)(b >> 32), src, 64);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ cd shfl_cd(cd v, int src) { return cmk(shfl_f64(v.x, src), shfl_f64(v.y, src)); }
 // component-wise select (a select of the vector type itself goes through memory)
 __device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cmk(c ? a.x : b.x, c ? a.y : b.y); }
 

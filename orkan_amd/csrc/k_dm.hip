@@ -53,6 +53,7 @@ struct DmOp {
     int ph[8];      // MONO: U|a> = i^ph[a] |perm a>
     cd m[64];       // SOP: 4x4 row-major; DENSE: 2^k x 2^k row-major
     const cd *s2;   // SOP2: 16x16 row-major superoperator in device memory
+    u64 still;      // MONO: bit a * 8 + b set when entries with gate bits (a, b) keep their value
 };
 
 __device__ __forceinline__ u64 spread_bits(int a, const int *pos, int k) {
@@ -119,6 +120,17 @@ __device__ __forceinline__ bool tile_stored(const DmPlan &p, u64 TR, u64 TC) {
     else return (TR >> (p.L - p.LB)) >= (TC >> (p.L - p.LB));
 }
 
+// does LDS entry (x, y) change?  Only a permutation leaves entries as they are.
+template <int OP>
+__device__ __forceinline__ bool moved(const DmOp &op, int x, int y) {
+    if constexpr (OP == ORK_DM_MONO) {
+        const int a = gather_bits((u64)x, op.lbit, op.k), b = gather_bits((u64)y, op.lbit, op.k);
+        return !((op.still >> (a * 8 + b)) & 1);
+    } else {
+        return true;
+    }
+}
+
 // LDS entry (x, y) of the updated group; permutations with phases are applied
 // here as a gather: rho'(perm a, perm b) = i^(ph[a] - ph[b]) rho(a, b)
 template <int OP>
@@ -135,9 +147,12 @@ __device__ __forceinline__ cd lds_get(const cd *lds, const DmOp &op, int x, int 
     }
 }
 
-template <typename S, int LAYOUT, int OP>
+// HOIST: every load of a thread in flight before the first LDS write (4 per thread, at the cost of
+// registers), else one load after the other at the occupancy the smaller register count allows
+template <typename S, int LAYOUT, int OP, int HOIST>
 __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rho, DmPlan p, DmOp op) {
     __shared__ cd lds[W_MAX * LDS_STRIDE];
+    constexpr int EPT = W_MAX * W_MAX / DM_THREADS;   // entries of a group per thread
     const int LB = p.LB, g = p.g, LW = p.LW;
     const int B = 1 << LB, W = 1 << LW, NT = 1 << g;
     const int total = W * W;
@@ -150,23 +165,66 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
         const bool diag_group = (R == C);
 
         // ---------------- load ----------------
-        for (int t = tid; t < total; t += DM_THREADS) {
-            const int ti = t >> (2 * LB), e = t & (B * B - 1);
-            const int i = ti >> g, j = ti & (NT - 1);
-            const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
-            const int sa = e >> LB, sb = e & (B - 1);                  // slow, fast
-            const int sr = (LAYOUT == ORK_TILED) ? sa : sb;            // row-major vs column-major
-            const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
-            if (tile_stored<LAYOUT>(p, TR, TC)) {
-                if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
-                const cd v = ldc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
-                lds[(i * B + sr) * LDS_STRIDE + j * B + sc] = v;
-                if (LAYOUT == ORK_PACKED && TR == TC && sr > sc)
+        // for a permutation, entries that keep their value are neither loaded nor stored (nothing reads
+        // them: the sources of the moved entries are the moved entries)
+        if constexpr (HOIST) {
+            cd lv[EPT];
+            int lx[EPT], ly[EPT], lk[EPT];   // LDS position; 0 skip, 1 direct, 2 direct + transposed copy, 3 mirror
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                const int t = tid + u * DM_THREADS;
+                lk[u] = 0;
+                lx[u] = ly[u] = 0;
+                lv[u] = cmk(0.0, 0.0);
+                if (t >= total) continue;
+                const int ti = t >> (2 * LB), e = t & (B * B - 1);
+                const int i = ti >> g, j = ti & (NT - 1);
+                const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
+                const int sa = e >> LB, sb = e & (B - 1);                  // slow, fast
+                const int sr = (LAYOUT == ORK_TILED) ? sa : sb;            // row-major vs column-major
+                const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
+                if (tile_stored<LAYOUT>(p, TR, TC)) {
+                    if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
+                    lx[u] = i * B + sr;
+                    ly[u] = j * B + sc;
+                    if (!moved<OP>(op, lx[u], ly[u])) continue;
+                    lk[u] = (LAYOUT == ORK_PACKED && TR == TC && sr > sc) ? 2 : 1;
+                    lv[u] = ldc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
+                } else {
+                    // stored mirror tile (TC, TR): its entry (sr, sc) is conj of ours at (sc, sr)
+                    lx[u] = i * B + sc;
+                    ly[u] = j * B + sr;
+                    if (!moved<OP>(op, lx[u], ly[u])) continue;
+                    lk[u] = 3;
+                    lv[u] = ldc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                if (!lk[u]) continue;
+                lds[lx[u] * LDS_STRIDE + ly[u]] = lk[u] == 3 ? cconj(lv[u]) : lv[u];
+                if (lk[u] == 2) lds[ly[u] * LDS_STRIDE + lx[u]] = cconj(lv[u]);
+            }
+        } else {
+            for (int t = tid; t < total; t += DM_THREADS) {
+                const int ti = t >> (2 * LB), e = t & (B * B - 1);
+                const int i = ti >> g, j = ti & (NT - 1);
+                const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
+                const int sa = e >> LB, sb = e & (B - 1);
+                const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
+                const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
+                if (tile_stored<LAYOUT>(p, TR, TC)) {
+                    if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
+                    if (!moved<OP>(op, i * B + sr, j * B + sc)) continue;
+                    const cd v = ldc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
+                    lds[(i * B + sr) * LDS_STRIDE + j * B + sc] = v;
+                    if (LAYOUT == ORK_PACKED && TR == TC && sr > sc)
+                        lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
+                } else {
+                    if (!moved<OP>(op, i * B + sc, j * B + sr)) continue;
+                    const cd v = ldc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
                     lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
-            } else {
-                // stored mirror tile (TC, TR): its entry (sr, sc) is conj of ours at (sc, sr)
-                const cd v = ldc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
-                lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
+                }
             }
         }
         __syncthreads();
@@ -276,8 +334,9 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
             const int sa = e >> LB, sb = e & (B - 1);
             const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
             const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
-            const bool direct = tile_stored<LAYOUT>(p, TR, TC) && !(LAYOUT == ORK_PACKED && TR == TC && sr < sc);
-            const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR);
+            const bool direct = tile_stored<LAYOUT>(p, TR, TC) && !(LAYOUT == ORK_PACKED && TR == TC && sr < sc) &&
+                                moved<OP>(op, i * B + sr, j * B + sc);
+            const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR) && moved<OP>(op, i * B + sc, j * B + sr);
             if (direct) stc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, op, i * B + sr, j * B + sc));
             if (mirror) stc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, op, i * B + sc, j * B + sr)));
         }
@@ -570,6 +629,44 @@ struct PackedStreamPlan {
 
 __device__ __forceinline__ u64 col_base(u64 D, u64 c) { return c * (2 * D - c + 1) / 2; }   // offset of (c, c)
 
+// Is the 128 B line of stored entry (r, c) entirely inside the rows the workgroup of block row A owns in
+// column c?  Those rows are ins0(32 A .. 32 A + 31, q) and the same | 2^q: two runs of 32 rows for
+// q >= 5, one run of 64 for q < 5, cut at the diagonal.  A line that crosses a run end is shared
+// with another workgroup (the neighbouring block, or the previous column's last rows).
+template <typename S>
+__device__ __forceinline__ bool packed_line_owned(u64 D, int q, u64 A, u64 r, u64 c) {
+    constexpr int LG = sizeof(S) == 8 ? 4 : 3;   // log2 of the entries per line
+    u64 rs, re;
+    if (q >= 5) {
+        const u64 R1 = ins0(A * 32, q);
+        rs = r >= R1 + 32 ? R1 + ((u64)1 << q) : R1;
+        re = rs + 32;
+    } else {
+        rs = A * 64;
+        re = rs + 64;
+    }
+    if (rs < c) rs = c;
+    const u64 base = col_base(D, c) - c;   // offset of (r, c) is base + r
+    const u64 l0 = ((base + r) >> LG) << LG;
+    return l0 >= base + rs && l0 + ((u64)1 << LG) <= base + re;
+}
+
+// NT 2: nontemporal accesses on the lines a workgroup owns alone, cached ones on lines it shares (the
+// ends of its column runs) and on the mirror entries, so that the shared lines meet in L2 instead of
+// going to HBM twice.
+template <int NT, typename S> __device__ __forceinline__ cd ldh(const S *p, bool own) {
+    if constexpr (NT == 2) return own ? ldx<1>(p) : ldx<0>(p);
+    else return ldx<NT>(p);
+}
+template <int NT, typename S> __device__ __forceinline__ void sth(S *p, cd v, bool own) {
+    if constexpr (NT == 2) {
+        if (own) stx<1>(p, v);
+        else stx<0>(p, v);
+    } else {
+        stx<NT>(p, v);
+    }
+}
+
 // PARTS consecutive workgroups (after the XCD remap) share one 32 x 32 block, each taking 32 / PARTS
 // of its columns b (their quads and their mirror entries): no entry is touched by two of them.
 template <typename S, int NT, int PARTS>
@@ -593,7 +690,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             const int t = tid + DM_THREADS * h, x = t / YW, y = y0 + t % YW;   // lanes walk y: rows c1 of column r0(x)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
-                mv[h] = ldx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
+                mv[h] = ldx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
     }
     // all quads of a thread in flight: t = tid + 256 h
@@ -601,12 +698,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
         cd v[QPT][4];
         S *p0[QPT], *p1[QPT], *p2[QPT], *p3[QPT];
         int kind[QPT];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
+        int own[QPT];    // bit j: the line of p_j is the workgroup's alone (NT 2)
 #pragma unroll
         for (int h = 0; h < QPT; ++h) {
             const int tt = tid + DM_THREADS * h, x = tt & 31, y = y0 + (tt >> 5);   // lanes walk x: rows r0 of column c0(y)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             kind[h] = a < b ? 0 : a == b ? 3 : ((a >> q) == (b >> q) ? 2 : 1);
             p0[h] = p1[h] = p2[h] = p3[h] = rho;
+            own[h] = 0;
             if (kind[h]) {
                 const u64 r0 = ins0(a, q), c0 = ins0(b, q), r1 = r0 | m, c1 = c0 | m;
                 const u64 k0 = col_base(D, c0), k1 = col_base(D, c1);
@@ -614,10 +713,15 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
                 p2[h] = rho + k0 + (r1 - c0);
                 p3[h] = rho + k1 + (r1 - c1);
                 if (kind[h] == 1) p1[h] = rho + k1 + (r0 - c1);   // (r0, c1) is stored only here
-                v[h][0] = ldx<NT>(p0[h]);
-                v[h][2] = ldx<NT>(p2[h]);
-                v[h][3] = ldx<NT>(p3[h]);
-                if (kind[h] == 1) v[h][1] = ldx<NT>(p1[h]);
+                if constexpr (NT == 2) {
+                    own[h] = (int)packed_line_owned<S>(D, q, A, r0, c0) | (int)packed_line_owned<S>(D, q, A, r1, c0) << 2 |
+                             (int)packed_line_owned<S>(D, q, A, r1, c1) << 3;
+                    if (kind[h] == 1) own[h] |= (int)packed_line_owned<S>(D, q, A, r0, c1) << 1;
+                }
+                v[h][0] = ldh<NT>(p0[h], own[h] & 1);
+                v[h][2] = ldh<NT>(p2[h], (own[h] >> 2) & 1);
+                v[h][3] = ldh<NT>(p3[h], (own[h] >> 3) & 1);
+                if (kind[h] == 1) v[h][1] = ldh<NT>(p1[h], (own[h] >> 1) & 1);
             }
         }
         if (has_mirror) {
@@ -637,10 +741,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             else if (kind[h] == 3) v[h][1] = cconj(v[h][2]);
             cd o0, o1, o2, o3;
             sop4(op, v[h][0], v[h][1], v[h][2], v[h][3], o0, o1, o2, o3);
-            stx<NT>(p0[h], o0);
-            stx<NT>(p2[h], o2);
-            stx<NT>(p3[h], o3);
-            if (kind[h] == 1) stx<NT>(p1[h], o1);
+            sth<NT>(p0[h], o0, own[h] & 1);
+            sth<NT>(p2[h], o2, (own[h] >> 2) & 1);
+            sth<NT>(p3[h], o3, (own[h] >> 3) & 1);
+            if (kind[h] == 1) sth<NT>(p1[h], o1, (own[h] >> 1) & 1);
             else if (kind[h] == 2) ms[x * LDS_STRIDE + y] = cconj(o1);
         }
     }
@@ -650,15 +754,156 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             const int x = t / YW, y = y0 + t % YW;
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             if (a > b && (a >> q) == (b >> q))
-                stx<NT>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
+                stx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
+        }
+    }
+}
+
+// ---- permutations of whole tiles: every gate qubit at or above the tile -----------------------
+// When every qubit of a permutation gate lies at or above the tile (tiled: the storage tile, q >= L;
+// packed: 32 x 32 blocks, q >= 5), the gate maps whole tiles onto whole tiles: the new tile (TR, TC)
+// is i^(ph[pinv a] - ph[pinv b]) times the old logical tile (P TR, P TC), with a, b the gate bits of
+// TR, TC and P the permutation of tile indices; in-tile positions stay.  For the involutions of the
+// gate set (cx, cy, swap, ccx) the stored tiles pair up: s and its source p = fold(P TR, P TC), the
+// stored tile holding the source (its conjugate transpose when (P TR, P TC) lies above the diagonal).
+// One workgroup per stored tile s; it works only when s is not after p in storage order, then swaps
+// both tiles as two coalesced streams (through LDS for a transpose).  Diagonal tiles map onto diagonal
+// tiles.  Tiles whose value does not change are neither read nor written.  (Emulated on the CPU against
+// U rho U^dagger for every gate and qubit order above the tile, both layouts, every entry written at
+// most once, before the first GPU run.)
+struct PermTilePlan {
+    u64 ntiles;   // stored tiles (tile rows nt: nt (nt + 1) / 2)
+    u64 D;        // dimension (packed offsets)
+    int L;        // tile log: storage tile (tiled), 5 (packed)
+    int k;
+    int pos[3];   // tile-index bit of gate argument j (q_j - L)
+    int pinv[8];  // gate bits a take the tile whose gate bits are pinv[a] ...
+    int ph[8];    // ... times i^(ph[pinv a] - ph[pinv b])
+    int xcd;
+};
+
+__device__ __forceinline__ u64 perm_src_tile(const PermTilePlan &p, u64 t, int &a_out) {
+    int a = 0;
+    for (int j = 0; j < p.k; ++j) a |= (int)((t >> p.pos[j]) & 1) << j;
+    const int s = p.pinv[a];
+    for (int j = 0; j < p.k; ++j)
+        if (((a ^ s) >> j) & 1) t ^= (u64)1 << p.pos[j];
+    a_out = s;
+    return t;
+}
+
+// entry o (memory order) of tile (TR, TC): its in-tile position (i, j), offset, and whether it is stored
+template <int LAYOUT>
+__device__ __forceinline__ void perm_tile_entry(const PermTilePlan &p, u64 TR, u64 TC, int o, int &i, int &j, u64 &off,
+                                                bool &st) {
+    if constexpr (LAYOUT == ORK_TILED) {
+        i = o >> p.L;
+        j = o & ((1 << p.L) - 1);
+        off = tile_base(TR, TC, p.L) + (u64)o;
+        st = true;
+    } else {
+        j = o >> 5;
+        i = o & 31;
+        const u64 r = TR * 32 + (u64)i, c = TC * 32 + (u64)j;
+        st = r >= c;
+        off = st ? col_base(p.D, c) + (r - c) : 0;
+    }
+}
+
+// TRANS 0: the tile pairs whose source lies below the diagonal (and tiles that only change phase), with
+// no LDS; PARTS consecutive workgroups share a pair, each taking 1 / PARTS of both tiles in memory order
+// (the same positions in s and p).  TRANS 1: the pairs whose source is a conjugate transpose, whole in
+// one workgroup through LDS (a split would let one workgroup read entries another has rewritten).
+template <typename S, int LAYOUT, int PARTS, int TRANS>
+__global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict__ rho, PermTilePlan p) {
+    constexpr int E = W_MAX * W_MAX / DM_THREADS / PARTS;   // entries of each tile per thread
+    const int T = LAYOUT == ORK_TILED ? (1 << p.L) : 32, TT = T * T, tid = threadIdx.x;
+    const u64 g = remap_group(blockIdx.x, p.ntiles * PARTS, p.xcd);
+    const u64 w = g / PARTS;
+    const int part = (int)(g % PARTS);
+    if (w >= p.ntiles) return;
+    u64 TR, TC;
+    tri_decode(w, TR, TC);
+    int sa, sb;
+    const u64 PR = perm_src_tile(p, TR, sa), PC = perm_src_tile(p, TC, sb);
+    const bool trans = PR < PC;
+    if (trans != (bool)TRANS) return;
+    const u64 pr = trans ? PC : PR, pc = trans ? PR : PC;
+    const u64 wp = pr * (pr + 1) / 2 + pc;
+    if (wp < w) return;   // the workgroup of p swaps both
+    const int phs = (p.ph[sa] - p.ph[sb]) & 3;
+    if (wp == w && !trans && phs == 0) return;   // the tile keeps its value
+    int pa, pb;
+    (void)perm_src_tile(p, pr, pa);
+    (void)perm_src_tile(p, pc, pb);
+    const int php = (p.ph[pa] - p.ph[pb]) & 3;
+    const bool pair = wp != w;
+    const int o0 = part * (TT / PARTS);
+    cd vs[E], vp[E];
+    u64 os[E], op[E];
+    bool ok[E];
+    int ii[E], jj[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int o = o0 + tid + e * DM_THREADS;
+        bool st = false;
+        int i = 0, j = 0;
+        u64 off = 0, offp = 0;
+        if (o < o0 + TT / PARTS && o < TT) perm_tile_entry<LAYOUT>(p, TR, TC, o, i, j, off, st);
+        ok[e] = st;
+        ii[e] = i;
+        jj[e] = j;
+        os[e] = off;
+        if (st && pair) {
+            bool stp;
+            int i2, j2;
+            perm_tile_entry<LAYOUT>(p, pr, pc, o, i2, j2, offp, stp);
+        }
+        op[e] = offp;
+        vs[e] = st ? ldc(rho + off) : cmk(0.0, 0.0);
+        vp[e] = (st && pair) ? ldc(rho + offp) : cmk(0.0, 0.0);
+    }
+    if constexpr (!TRANS) {
+        // same in-tile positions: s takes p's entries and p takes s's (or s its own, phase only)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (!ok[e]) continue;
+            stc(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
+            if (pair) stc(rho + op[e], cipow(vs[e], php));
+        }
+    } else {
+        // new s (i, j) = i^phs conj(old p (j, i)), new p (i, j) = i^php conj(old s (j, i)); off the
+        // diagonal every entry of both tiles is stored
+        __shared__ cd ts[W_MAX * LDS_STRIDE];
+        __shared__ cd tp[W_MAX * LDS_STRIDE];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (!ok[e]) continue;
+            ts[ii[e] * LDS_STRIDE + jj[e]] = vs[e];
+            tp[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (!ok[e]) continue;
+            stc(rho + os[e], cipow(cconj(tp[jj[e] * LDS_STRIDE + ii[e]]), phs));
+            if (pair) stc(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
         }
     }
 }
 
 // ---- elementwise phase kernels: rho_rc *= F(r) conj(F(c)) --------------------------------
-// Each workgroup takes one stored B x B tile (TR >= TC).  `mode` 0: gate phases
-// from a 2^k table with a per-pair skip mask; mode 1: per-index phases (exp_diag),
-// diagonal entries untouched.
+// The stored entries are streamed in memory order as one flat array: workgroup w takes the
+// E * 256 consecutive entries from w * E * 256 on, thread t the entries w * E * 256 + t + 256 j
+// (j < E), so every load and store instruction covers consecutive addresses.  Each thread decodes
+// (r, c) of its entries from their flat index (tiled: tile index, then row-major inside the tile;
+// packed: the column-major triangle), issues all its loads, then multiplies and stores.
+// MODE 0: gate phases from a 2^k table with a per-pair skip mask -- entries whose factor is exactly 1
+// are neither read nor written; MODE 1: per-index phases (exp_diag).  Diagonal entries are never
+// changed (src/circ/circ_packed.c:28-71, src/gate/tiled/gate_tiled_cz.c:32-525 touch only the
+// entries whose factor differs from 1).  WL (whole lines): a 128 B line holding any changed entry is
+// read and written in full, its other entries written back exactly as read, so that HBM sees
+// whole-line writes where the changed entries do not fill lines (a gate qubit below 3 in complex128).
 struct PhaseArgs {
     int k;
     int qs[3];
@@ -666,34 +911,104 @@ struct PhaseArgs {
     cd f[64];           // f[a * 8 + b] = F[a] conj(F[b])
 };
 
-template <typename S, int LAYOUT, int MODE, int NT>
-__global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rho, DmPlan p, PhaseArgs a,
-                                                              const double2 *__restrict__ cs) {
-    const int LB = p.LB, B = 1 << LB;
-    for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
-        u64 TR, TC;
-        group_rc(p, w0, TR, TC);
-        for (int e = threadIdx.x; e < B * B; e += DM_THREADS) {
-            const int sa = e >> LB, sb = e & (B - 1);
-            const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
-            const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
-            const u64 r = TR * B + sr, c = TC * B + sc;
-            if (LAYOUT == ORK_PACKED && r < c) continue;
-            if (r == c) continue;
-            S *ptr = rho + elem_off<LAYOUT>(p, r, c);
-            if constexpr (MODE == 0) {
-                const int ar = gather_bits(r, a.qs, a.k), ac = gather_bits(c, a.qs, a.k);
-                if (a.skip[ar] >> ac & 1) continue;
-                stx<NT>(ptr, cmul(a.f[ar * 8 + ac], ldx<NT>(ptr)));
-            } else {
-                // e^{-i(d_r - d_c)t} = (cr - i sr)(cc + i sc)   (src/circ/circ_packed.c:54-60)
-                const double2 pr = cs[r], pc = cs[c];
-                const double fre = pr.x * pc.x + pr.y * pc.y;
-                const double fim = -pr.y * pc.x + pr.x * pc.y;
-                const cd v = ldx<NT>(ptr);
-                stx<NT>(ptr, cmk(fre * v.x - fim * v.y, fre * v.y + fim * v.x));
-            }
+struct PhasePlan {
+    u64 dim;       // D = 2^n
+    u64 len;       // stored entries
+    u64 nchunks;   // workgroups of the launch
+    int L;         // tiled: tiles of 2^L x 2^L (one zero-padded tile when n < L)
+    int xcd;
+    int tile_exit; // tiled, one tile per workgroup: leave tiles without a changed entry at once
+    u64 tchg;      // bit a * 8 + b: a tile whose gate bits above the tile are (a, b) holds a changed entry
+};
+
+// (r, c) of flat index x; the cache holds the last tile (tiled: key = tile, P, Q = TR, TC) or column
+// (packed: key = offset of the next column, P = column, Q = offset of its diagonal entry)
+template <int LAYOUT>
+__device__ __forceinline__ void flat_rc(const PhasePlan &pp, u64 x, u64 &key, u64 &P, u64 &Q, u64 &r, u64 &c) {
+    if constexpr (LAYOUT == ORK_TILED) {
+        const int L = pp.L;
+        const u64 t = x >> (2 * L), o = x & ((((u64)1) << (2 * L)) - 1);
+        if (t != key) {
+            tri_decode(t, P, Q);
+            key = t;
         }
+        r = (P << L) + (o >> L);
+        c = (Q << L) + (o & ((((u64)1) << L) - 1));
+    } else {
+        const u64 D = pp.dim;
+        if (x >= key || x < Q) {
+            // column c: c (2D - c + 1) / 2 <= x < (c + 1) (2D - c) / 2
+            const double b = 2.0 * (double)D + 1.0;
+            u64 cc = (u64)((b - sqrt(b * b - 8.0 * (double)x)) * 0.5);
+            if (cc >= D) cc = D - 1;
+            while (cc > 0 && col_base(D, cc) > x) --cc;
+            while (cc + 1 < D && col_base(D, cc + 1) <= x) ++cc;
+            P = cc;
+            Q = col_base(D, cc);
+            key = Q + (D - cc);
+        }
+        c = P;
+        r = P + (x - Q);
+    }
+}
+
+template <typename S, int LAYOUT, int MODE, int NT, int WL, int E>
+__global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rho, PhasePlan pp, PhaseArgs a,
+                                                              const double2 *__restrict__ cs) {
+    constexpr int G = 128 / (int)sizeof(S);   // entries per 128 B line: lanes lane & ~(G - 1) .. + G - 1
+    const u64 w = remap_group(blockIdx.x, pp.nchunks, pp.xcd);
+    const u64 x0 = w * (u64)(E * DM_THREADS) + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    u64 key = ~(u64)0, P = 0, Q = 0;
+    if (LAYOUT == ORK_PACKED) key = 0;
+    if constexpr (MODE == 0 && LAYOUT == ORK_TILED) {
+        if (pp.tile_exit) {
+            // the workgroup's tile w: its gate bits above the tile alone decide whether any entry changes
+            key = w;
+            tri_decode(w, P, Q);
+            const int at = gather_bits(P << pp.L, a.qs, a.k), bt = gather_bits(Q << pp.L, a.qs, a.k);
+            if (!((pp.tchg >> (at * 8 + bt)) & 1)) return;
+        }
+    }
+    cd f[E];
+    bool chg[E], act[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const u64 x = x0 + (u64)j * DM_THREADS;
+        bool ok = x < pp.len;
+        u64 r = 0, c = 0;
+        if (ok) flat_rc<LAYOUT>(pp, x, key, P, Q, r, c);
+        ok = ok && r < pp.dim && c < pp.dim && r != c;   // padding of a one-tile state; the diagonal
+        f[j] = cmk(1.0, 0.0);
+        if constexpr (MODE == 0) {
+            const int ar = gather_bits(r, a.qs, a.k), ac = gather_bits(c, a.qs, a.k);
+            ok = ok && !(a.skip[ar] >> ac & 1);
+            if (ok) f[j] = a.f[ar * 8 + ac];
+        } else if (ok) {
+            // e^{-i(d_r - d_c)t} = (cr - i sr)(cc + i sc)   (src/circ/circ_packed.c:54-60)
+            const double2 pr = cs[r], pc = cs[c];
+            f[j] = cmk(pr.x * pc.x + pr.y * pc.y, -pr.y * pc.x + pr.x * pc.y);
+        }
+        chg[j] = ok;
+        if constexpr (WL) {
+            const unsigned long long m = __ballot(ok);
+            act[j] = ((m >> (lane & ~(G - 1))) & ((1ull << G) - 1)) != 0;
+        } else {
+            act[j] = ok;
+        }
+    }
+    cd v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = act[j] ? ldx<NT>(rho + x0 + (u64)j * DM_THREADS) : cmk(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        if (!act[j]) continue;
+        cd o = v[j];
+        if (chg[j]) {
+            if constexpr (MODE == 0) o = cmul(f[j], v[j]);
+            else o = cmk(f[j].x * v[j].x - f[j].y * v[j].y, f[j].x * v[j].y + f[j].y * v[j].x);
+        }
+        stx<NT>(rho + x0 + (u64)j * DM_THREADS, o);
     }
 }
 
@@ -786,13 +1101,12 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 // the ends of a column run meet in one L2 (without it 1.33x the algorithmic
 // bytes are fetched); tiled: row order with the remap.
 // nt: nontemporal loads and stores; lp: log2 of the workgroups sharing one tile quad (cross-tile kernel)
-struct Shape { int xcd, col, lds, nt, lp; };
+// wl: whole-line writes of the phase kernels (flags bit 6)
+struct Shape { int xcd, col, lds, nt, lp, wl; };
 constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 0}};   // [packed, tiled]
-// phase kernels (cz, exp_diag, ...) measured on 14 qubits: packed best in column order, tiled in plain
-// order with nontemporal accesses where the changed entries fill whole 128 B lines (cz from qubits
-// (3, 4) up: +4 % over the neighbour pairs; on the packed layout they lose 5-15 %); both with XCD runs
-// of 32 groups (+2 %)
-constexpr Shape kPhaseShape[2] = {{2, 1, 0, 0}, {2, 0, 0, 1}};
+// phase kernels (cz, exp_diag, ...): flat streams of E entries per thread, XCD runs of 32 workgroups,
+// nontemporal accesses, whole-line writes where the changed entries do not fill lines
+constexpr Shape kPhaseShape[2] = {{2, 0, 0, 1, 0, 1}, {2, 0, 0, 1, 0, 1}};
 Shape g_group_shape[2] = {kGroupShape[0], kGroupShape[1]};
 Shape g_phase_shape[2] = {kPhaseShape[0], kPhaseShape[1]};
 
@@ -808,18 +1122,27 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     p.xcd = sh.xcd;
     p.col = sh.col;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
+    // bit 6 of the group shape flags (the wl field): every load of a thread in flight at once
+#define ORK_GROUP_LAUNCH(OPC)                                                                            \
+    do {                                                                                                 \
+        if (sh.wl) dm_group_kernel<S, LAYOUT, OPC, 1><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); \
+        else dm_group_kernel<S, LAYOUT, OPC, 0><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);       \
+    } while (0)
     switch (opk) {
-    case ORK_DM_SOP: dm_group_kernel<S, LAYOUT, ORK_DM_SOP><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
-    case ORK_DM_MONO: dm_group_kernel<S, LAYOUT, ORK_DM_MONO><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
-    case ORK_DM_DENSE: dm_group_kernel<S, LAYOUT, ORK_DM_DENSE><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
-    case ORK_DM_SOP2: dm_group_kernel<S, LAYOUT, ORK_DM_SOP2><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); break;
+    case ORK_DM_SOP: ORK_GROUP_LAUNCH(ORK_DM_SOP); break;
+    case ORK_DM_MONO: ORK_GROUP_LAUNCH(ORK_DM_MONO); break;
+    case ORK_DM_DENSE: ORK_GROUP_LAUNCH(ORK_DM_DENSE); break;
+    case ORK_DM_SOP2: ORK_GROUP_LAUNCH(ORK_DM_SOP2); break;
     default: return (int)hipErrorInvalidValue;
     }
+#undef ORK_GROUP_LAUNCH
     ORK_LAUNCH_CHECK();
     return 0;
 }
 
-// streamed tiled superoperator; shape [q < L, q >= L].  Measured on a 14-qubit density matrix
+// streamed tiled superoperator; shape [q < L, q >= L].  A later sweep of every target (h on 14 qubits)
+// moved the complex128 defaults to XCD runs of 32 workgroups under an 80 KB cap for in-tile targets 3-4
+// (6.74 against 6.58 TB/s) and a 64 KB cap for the split tile quads (6.36 against 6.23).  Measured on a 14-qubit density matrix
 // (32 x 32 tiles): in-tile targets stream best like the statevector kernels, with consecutive
 // chunks on one XCD and residency capped by 64 KB of LDS per workgroup (6.1 TB/s against 5.6 TB/s
 // uncapped); tile quads best split over 4 workgroups (8 rows of each tile, 16 KB per workgroup), in
@@ -834,7 +1157,7 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
 // Bit 3 (lp) of an in-tile shape enables the lane-exchange kernel for targets whose quad columns
 // share 128 B lines; that kernel runs with kLaneShape (in plain order, 48 KB cap for 8 entries per
 // thread: 6.41 against 6.01 TB/s for complex128 targets 0-2, 6.18 against 5.54 for complex64 0-3).
-constexpr Shape kStreamShape[4] = {{1, 0, 65536, 1, 1}, {0, 0, 49152, 1, 2}, {0, 0, 65536, 1, 1}, {0, 0, 49152, 1, 1}};
+constexpr Shape kStreamShape[4] = {{2, 0, 81920, 1, 1}, {0, 0, 65536, 1, 2}, {0, 0, 65536, 1, 1}, {0, 0, 49152, 1, 1}};
 constexpr Shape kLaneShape = {0, 0, 49152, 1, 0};
 Shape g_stream_shape[4] = {kStreamShape[0], kStreamShape[1], kStreamShape[2], kStreamShape[3]};
 int g_stream_on = 1;
@@ -922,7 +1245,13 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     if ((pp.nblocks << lp) > 0x7fffffffu) return (int)hipErrorInvalidValue;
     const unsigned nb = (unsigned)(pp.nblocks << lp);
     S *r = (S *)rho;
-    if (sh.nt) {
+    // bit 6 (the wl field) with nontemporal accesses: only on the lines a workgroup owns alone
+    const bool hybrid = sh.nt && sh.wl;
+    if (hybrid) {
+        if (lp == 2) dm_sop_packed_kernel<S, 2, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 2, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 2, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+    } else if (sh.nt) {
         if (lp == 2) dm_sop_packed_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
         else if (lp == 1) dm_sop_packed_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
         else dm_sop_packed_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
@@ -935,36 +1264,137 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     return 0;
 }
 
-template <int MODE>
-int launch_phase(hipStream_t st, int prec, int layout, void *rho, DmPlan p, const PhaseArgs &a,
-                 const double2 *cs) {
-    const unsigned grid = grid_for(p.ngroups);
-    Shape sh = g_phase_shape[layout == ORK_TILED ? 1 : 0];
-    p.xcd = sh.xcd;
-    p.col = sh.col;
-    const int lds = sh.lds;
-    const u64 elem = prec == ORK_F32 ? 8 : 16;
-    if (p.dim * (p.dim + 1) / 2 * elem < ork_nt_min_bytes()) sh.nt = 0;
-    if (MODE == 0) {   // gate phases: the lowest gate qubit decides whether changed entries fill lines
-        int lo = 7;   // past a 128 B line for either storage
-        for (int i = 0; i < a.k; ++i) lo = a.qs[i] < lo ? a.qs[i] : lo;
-        if ((elem << lo) < 128) sh.nt = 0;
+// whole-tile permutations: XCD runs of 32 workgroups, one workgroup per tile pair (splitting a pair over 2
+// or 4 lost 4-25 % on 14 qubits); `col` set switches the kernel off (tile groups instead); set through
+// orkan_set_launch_shape 22
+constexpr Shape kPermTileShape = {2, 0, 0, 0, 0, 0};
+Shape g_permt_shape = kPermTileShape;
+
+// can the gate run through dm_perm_tiles_kernel?  An involution whose qubits all lie at or above the
+// tile (log2 T: the storage tile, or 32 x 32 blocks for packed)
+bool perm_tiles_ok(int layout, int L, int n, const ork_mono_t *m) {
+    if (g_permt_shape.col) return false;
+    const int T = layout == ORK_TILED ? L : 5;
+    if (n <= T) return false;
+    for (int j = 0; j < m->k; ++j)
+        if (m->qs[j] < T) return false;
+    for (int a = 0; a < (1 << m->k); ++a)
+        if (m->perm[m->perm[a]] != a) return false;
+    return true;
+}
+
+template <typename S, int LAYOUT>
+int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t *m) {
+    PermTilePlan p;
+    p.D = (u64)1 << n;
+    p.L = LAYOUT == ORK_TILED ? L : 5;
+    const u64 nt = p.D >> p.L;
+    p.ntiles = nt * (nt + 1) / 2;
+    if (p.ntiles > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    p.k = m->k;
+    for (int j = 0; j < 3; ++j) p.pos[j] = j < m->k ? m->qs[j] - p.L : 0;
+    for (int a = 0; a < 8; ++a) { p.pinv[a] = a; p.ph[a] = 0; }
+    for (int a = 0; a < (1 << m->k); ++a) {
+        p.pinv[m->perm[a]] = a;
+        p.ph[a] = m->ph[a] & 3;
     }
-    if (sh.nt) {
-        if (prec == ORK_F32) {
-            if (layout == ORK_PACKED) dm_phase_kernel<float2, ORK_PACKED, MODE, 1><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
-            else dm_phase_kernel<float2, ORK_TILED, MODE, 1><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
+    const Shape sh = g_permt_shape;
+    p.xcd = sh.xcd;
+    // direct pairs split over 1 << lp workgroups (32 x 32 tiles only), then the transposed pairs
+    const int lp = (LAYOUT == ORK_PACKED || L == 5) ? sh.lp : 0;
+    const unsigned g0 = (unsigned)(p.ntiles << lp);
+    if ((p.ntiles << lp) > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    S *r = (S *)rho;
+    if (lp == 2) dm_perm_tiles_kernel<S, LAYOUT, 4, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
+    else if (lp == 1) dm_perm_tiles_kernel<S, LAYOUT, 2, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
+    else dm_perm_tiles_kernel<S, LAYOUT, 1, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
+    ORK_LAUNCH_CHECK();
+    dm_perm_tiles_kernel<S, LAYOUT, 1, 1><<<(unsigned)p.ntiles, DM_THREADS, 0, st>>>(r, p);
+    ORK_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename S, int LAYOUT, int MODE, int NT, int WL>
+void launch_phase_t(hipStream_t st, unsigned grid, int lds, void *rho, const PhasePlan &pp, const PhaseArgs &a,
+                    const double2 *cs) {
+    constexpr int E = sizeof(S) == 8 ? 8 : 4;   // 64 B of each thread in flight
+    dm_phase_kernel<S, LAYOUT, MODE, NT, WL, E><<<grid, DM_THREADS, lds, st>>>((S *)rho, pp, a, cs);
+}
+
+template <typename S, int MODE, int NT, int WL>
+void launch_phase_l(hipStream_t st, int layout, unsigned grid, int lds, void *rho, const PhasePlan &pp,
+                    const PhaseArgs &a, const double2 *cs) {
+    if (layout == ORK_PACKED) launch_phase_t<S, ORK_PACKED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs);
+    else launch_phase_t<S, ORK_TILED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs);
+}
+
+template <int MODE>
+int launch_phase(hipStream_t st, int prec, int layout, int L, int n, void *rho, const PhaseArgs &a,
+                 const double2 *cs) {
+    PhasePlan pp;
+    pp.dim = (u64)1 << n;
+    pp.L = L;
+    if (layout == ORK_PACKED) {
+        pp.len = pp.dim * (pp.dim + 1) / 2;
+    } else {
+        const u64 nt = n >= L ? pp.dim >> L : 1;
+        pp.len = (nt * (nt + 1) / 2) << (2 * L);
+    }
+    Shape sh = g_phase_shape[layout == ORK_TILED ? 1 : 0];
+    const u64 elem = prec == ORK_F32 ? 8 : 16;
+    const u64 per = (prec == ORK_F32 ? 8 : 4) * (u64)DM_THREADS;
+    pp.nchunks = (pp.len + per - 1) / per;
+    if (pp.nchunks > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    pp.xcd = sh.xcd;
+    if (pp.len * elem < ork_nt_min_bytes()) sh.nt = 0;
+    int wl = 0;
+    if (MODE == 0) {
+        // tiled: the lowest gate qubit decides whether changed entries fill 128 B lines (tile rows are
+        // line-aligned); packed: column runs start anywhere in a line, so any line may be partial
+        int lo = 7;     // past a 128 B line for either storage
+        for (int i = 0; i < a.k; ++i) lo = a.qs[i] < lo ? a.qs[i] : lo;
+        if ((elem << lo) < 128 || layout == ORK_PACKED) {
+            wl = sh.wl;
+            if (!wl) sh.nt = 0;   // partial-line writes stay cached
+        }
+    }
+    // tiled, one tile per workgroup: a mask over the gate bits above the tile of the tiles that hold a
+    // changed entry (any in-tile combination of the other gate bits not skipped)
+    pp.tile_exit = 0;
+    pp.tchg = 0;
+    if (MODE == 0 && layout == ORK_TILED && n >= L && ((u64)1 << (2 * L)) == per) {
+        int hi = 0;   // gate arguments at or above the tile
+        for (int j = 0; j < a.k; ++j) hi |= (a.qs[j] >= L) << j;
+        for (int at = 0; at < (1 << a.k); ++at)
+            for (int bt = 0; bt < (1 << a.k); ++bt) {
+                if ((at & ~hi) || (bt & ~hi)) continue;
+                bool any = false;
+                for (int ai = 0; ai < (1 << a.k); ++ai)
+                    for (int bi = 0; bi < (1 << a.k); ++bi) {
+                        if ((ai & hi) || (bi & hi)) continue;
+                        if (!((a.skip[at | ai] >> (bt | bi)) & 1)) any = true;
+                    }
+                if (any) pp.tchg |= (u64)1 << (at * 8 + bt);
+            }
+        pp.tile_exit = 1;
+    }
+    const unsigned grid = (unsigned)pp.nchunks;
+    const int lds = sh.lds;
+    if (prec == ORK_F32) {
+        if (sh.nt) {
+            if (wl) launch_phase_l<float2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs);
+            else launch_phase_l<float2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs);
         } else {
-            if (layout == ORK_PACKED) dm_phase_kernel<double2, ORK_PACKED, MODE, 1><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
-            else dm_phase_kernel<double2, ORK_TILED, MODE, 1><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
+            if (wl) launch_phase_l<float2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs);
+            else launch_phase_l<float2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs);
         }
     } else {
-        if (prec == ORK_F32) {
-            if (layout == ORK_PACKED) dm_phase_kernel<float2, ORK_PACKED, MODE, 0><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
-            else dm_phase_kernel<float2, ORK_TILED, MODE, 0><<<grid, DM_THREADS, lds, st>>>((float2 *)rho, p, a, cs);
+        if (sh.nt) {
+            if (wl) launch_phase_l<double2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs);
+            else launch_phase_l<double2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs);
         } else {
-            if (layout == ORK_PACKED) dm_phase_kernel<double2, ORK_PACKED, MODE, 0><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
-            else dm_phase_kernel<double2, ORK_TILED, MODE, 0><<<grid, DM_THREADS, lds, st>>>((double2 *)rho, p, a, cs);
+            if (wl) launch_phase_l<double2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs);
+            else launch_phase_l<double2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs);
         }
     }
     ORK_LAUNCH_CHECK();
@@ -994,13 +1424,15 @@ extern "C" void ork_dm_reset_shapes(void) {
     g_stream_on = 1;
     g_pstream_shape = kPstreamShape;
     g_pstream_shape32 = kPstreamShape32;
+    g_permt_shape = kPermTileShape;
 }
 
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 63 || ((flags >> 3) & 3) > 2)
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 127 || ((flags >> 3) & 3) > 2)
         return (int)hipErrorInvalidValue;
-    // bit 5: XCD runs of 32 groups instead of contiguous ranges
-    const Shape sh{(flags & 32) ? 2 : (flags & 1), (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, (flags >> 3) & 3};
+    // bit 5: XCD runs of 32 groups instead of contiguous ranges; bit 6: whole-line writes (phase kernels)
+    const Shape sh{(flags & 32) ? 2 : (flags & 1), (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, (flags >> 3) & 3,
+                   (flags >> 6) & 1};
     switch (which) {
     case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
     case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;
@@ -1013,6 +1445,7 @@ extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
     case 8: g_stream_on = lds_bytes != 0; break;
     case 9: g_pstream_shape = sh; break;
     case 18: g_pstream_shape32 = sh; break;
+    case 22: g_permt_shape = sh; break;
     case 15: g_stream_shape[2] = sh; break;
     case 16: g_stream_shape[3] = sh; break;
     default: return (int)hipErrorInvalidValue;
@@ -1051,11 +1484,18 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
         for (int b = a + 1; b < k; ++b)
             if (op.srt[b] < op.srt[a]) { int t = op.srt[a]; op.srt[a] = op.srt[b]; op.srt[b] = t; }
     for (int a = 0; a < 8; ++a) { op.pinv[a] = a; op.ph[a] = 0; }
+    op.still = 0;
     if (opk == ORK_DM_MONO) {
         for (int a = 0; a < (1 << k); ++a) {
+            if (m->perm[a] < 0 || m->perm[a] >= (1 << k)) return (int)hipErrorInvalidValue;
             op.pinv[m->perm[a]] = a;
             op.ph[a] = m->ph[a] & 3;
         }
+        // entry (a, b) of a 2^k block receives rho(pinv a, pinv b) i^(ph[pinv a] - ph[pinv b])
+        for (int a = 0; a < (1 << k); ++a)
+            for (int b = 0; b < (1 << k); ++b)
+                if (op.pinv[a] == a && op.pinv[b] == b && ((op.ph[a] - op.ph[b]) & 3) == 0)
+                    op.still |= (u64)1 << (a * 8 + b);
     }
     const int nm = (opk == ORK_DM_SOP) ? 16 : (opk == ORK_DM_DENSE ? (1 << (2 * k)) : 0);
     for (int i = 0; i < 64; ++i) op.m[i] = make_double2(0.0, 0.0);
@@ -1063,6 +1503,13 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     op.s2 = opk == ORK_DM_SOP2 ? (const cd *)mat : nullptr;
     if (opk == ORK_DM_SOP2 && k != 2) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
+    if (opk == ORK_DM_MONO && perm_tiles_ok(layout, log_tile, n, m)) {
+        if (prec == ORK_F32)
+            return layout == ORK_PACKED ? launch_perm_tiles<float2, ORK_PACKED>(st, rho, n, log_tile, m)
+                                        : launch_perm_tiles<float2, ORK_TILED>(st, rho, n, log_tile, m);
+        return layout == ORK_PACKED ? launch_perm_tiles<double2, ORK_PACKED>(st, rho, n, log_tile, m)
+                                    : launch_perm_tiles<double2, ORK_TILED>(st, rho, n, log_tile, m);
+    }
     if (layout == ORK_PACKED && opk == ORK_DM_SOP && g_stream_on && n >= 6)
         return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, m->qs[0], op)
                                : launch_pstream<double2>(st, rho, n, m->qs[0], op);
@@ -1079,7 +1526,9 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
 
 extern "C" int ork_dm_phase(void *stream, int prec, int layout, int log_tile, void *rho, int n, int k, const int *qs,
                             const double *f) {
-    const DmPlan p = tile_plan(layout, log_tile, n);
+    if ((layout != ORK_PACKED && layout != ORK_TILED) || k < 1 || k > 3 || n < 1 || n > 31 || log_tile < 1 ||
+        log_tile > LW_MAX)
+        return (int)hipErrorInvalidValue;
     PhaseArgs a;
     a.k = k;
     for (int j = 0; j < 3; ++j) a.qs[j] = j < k ? qs[j] : 0;
@@ -1093,19 +1542,21 @@ extern "C" int ork_dm_phase(void *stream, int prec, int layout, int log_tile, vo
             if (re == 1.0 && im == 0.0) a.skip[x] |= 1u << y;
         }
     }
-    return launch_phase<0>((hipStream_t)stream, prec, layout, rho, p, a, nullptr);
+    return launch_phase<0>((hipStream_t)stream, prec, layout, log_tile, n, rho, a, nullptr);
 }
 
 extern "C" int ork_dm_exp_diag(void *stream, int prec, int layout, int log_tile, void *rho, int n,
                                const double *diag, double t, void *phase) {
-    const DmPlan p = tile_plan(layout, log_tile, n);
+    if ((layout != ORK_PACKED && layout != ORK_TILED) || n < 1 || n > 31 || log_tile < 1 || log_tile > LW_MAX)
+        return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
-    const unsigned g1 = (unsigned)((p.dim + DM_THREADS - 1) / DM_THREADS);
-    sincos_kernel<<<g1, DM_THREADS, 0, st>>>((double2 *)phase, diag, p.dim, t);
+    const u64 dim = (u64)1 << n;
+    const unsigned g1 = (unsigned)((dim + DM_THREADS - 1) / DM_THREADS);
+    sincos_kernel<<<g1, DM_THREADS, 0, st>>>((double2 *)phase, diag, dim, t);
     ORK_LAUNCH_CHECK();
     PhaseArgs a;
     a.k = 0;
-    return launch_phase<1>(st, prec, layout, rho, p, a, (const double2 *)phase);
+    return launch_phase<1>(st, prec, layout, log_tile, n, rho, a, (const double2 *)phase);
 }
 
 extern "C" int ork_dm_mean(void *stream, int prec, int layout, int log_tile, const void *rho, int n,

@@ -256,6 +256,98 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
     }
 }
 
+// Every gate qubit below 6: one amplitude per lane in memory order, so that every instruction covers
+// whole 128 B lines.  Lane l's new amplitude is i^ph[s] times the old amplitude of lane l ^ d, where the
+// gate bits a of l pick the source s = pinv[a] and d spreads a ^ s over the gate qubits: the gate bits
+// are lane bits because workgroups start on multiples of 256 amplitudes.  A line in which no amplitude
+// changes is neither read nor written (the sources of changed amplitudes are changed amplitudes); with
+// WL the other lines are written whole, unchanged amplitudes as read, else only changed amplitudes.
+struct PermLaneArgs {
+    int k;
+    int qs[3];
+    int pinv[8];   // gate bits a take the amplitude whose gate bits are pinv[a]
+    int ph[8];     // ... times i^ph[pinv[a]]
+};
+
+template <typename S, int U, int XCD, int NT, int WL>
+__global__ __launch_bounds__(SV_THREADS) void sv_perm_lane_kernel(S *__restrict__ psi, u64 dim, PermLaneArgs m) {
+    constexpr int G = 128 / (int)sizeof(S);   // amplitudes per line: lanes (lane & ~(G - 1)) .. + G - 1
+    const u64 base = chunk_of(XCD) * (SV_THREADS * U) + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int a = 0;
+    for (int j = 0; j < m.k; ++j) a |= ((lane >> m.qs[j]) & 1) << j;
+    const int s = m.pinv[a];
+    int d = 0;
+    for (int j = 0; j < m.k; ++j) d |= (((a ^ s) >> j) & 1) << m.qs[j];
+    const int src = lane ^ d, ph = m.ph[s] & 3;
+    const bool chg = (s != a) || ph != 0;
+    const unsigned long long cm = __ballot(chg);   // depends on the lane only, the same for every chunk
+    const bool line = ((cm >> (lane & ~(G - 1))) & ((1ull << G) - 1)) != 0;
+    const bool wr = WL ? line : chg;
+    cd v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 x = base + (u64)u * SV_THREADS;
+        v[u] = (line && x < dim) ? ldx<NT>(psi + x) : cmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 x = base + (u64)u * SV_THREADS;
+        const cd w = shfl_cd(v[u], src);   // every lane takes part
+        if (wr && x < dim) stx<NT>(psi + x, chg ? cipow(w, ph) : v[u]);
+    }
+}
+
+// Complex64 storage with no gate qubit at bit 0: amplitudes 2p and 2p + 1 move together, so the
+// permutation runs on 16 B elements (two amplitudes each, gate qubits one lower) -- one 16 B access
+// per two amplitudes instead of two 8 B accesses; the phases i^k act on each half exactly.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v cipow2(f4v a, int k) {
+    f4v r = a;
+    switch (k & 3) {
+    case 1: r.x = -a.y; r.y = a.x; r.z = -a.w; r.w = a.z; break;
+    case 2: r = -a; break;
+    case 3: r.x = a.y; r.y = -a.x; r.z = a.w; r.w = -a.z; break;
+    default: break;
+    }
+    return r;
+}
+template <int NT> __device__ __forceinline__ f4v ld4(const f4v *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <int NT> __device__ __forceinline__ void st4(f4v *p, f4v v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int U, int NT>
+__global__ __launch_bounds__(SV_THREADS) void sv_mono_x2_kernel(f4v *__restrict__ psi2, u64 nbase, MonoArgs m) {
+    const u64 b0 = chunk_of(1) * (SV_THREADS * U) + threadIdx.x;
+    f4v v[U][8];
+    u64 base[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 b = b0 + (u64)u * SV_THREADS;
+        base[u] = ins0k(b, m.pos, m.k);
+        if (b < nbase) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m.nt) v[u][j] = ld4<NT>(psi2 + (base[u] | m.src[j]));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const u64 b = b0 + (u64)u * SV_THREADS;
+        if (b < nbase) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m.nt) st4<NT>(psi2 + (base[u] | m.dst[j]), cipow2(v[u][j], m.ph[j]));
+        }
+    }
+}
+
 // ---- exp_diag: psi_x *= exp(-i diag[x] t) ------------------------------------------------
 template <typename S, int NT, int U>
 __global__ __launch_bounds__(SV_THREADS) void sv_exp_diag_kernel(S *__restrict__ psi, const double *__restrict__ diag,
@@ -684,6 +776,69 @@ extern "C" int ork_sv_1q(void *stream, int prec, void *psi, int n, int q, int op
 constexpr int kMonoLds32 = 26624;
 static int g_mono_lds32 = kMonoLds32;
 
+// permutation kernels: lane-exchange form (every gate qubit below 6, one of them inside a 128 B line)
+// and the paired complex64 form; set through orkan_set_launch_shape 21
+struct PermShape {
+    int lane_on, lane_all;     // lane kernel where a gate qubit lies inside a line / for every qubit set below 6
+    int lane_u, lane_u32;      // amplitudes per thread (2, 4 or 8)
+    int lane_lds, lane_lds32;  // residency caps
+    int wl;                    // whole-line writes
+    int x2_on, x2_lds;         // complex64: 16 B elements when no gate qubit is bit 0
+};
+// measured on 2^28 amplitudes (cx on every neighbour pair): complex128 lane kernel 4 amplitudes per thread
+// under a 40 KB cap (low pairs 3.24 TB/s against 2.66 through the per-group kernel), complex64 8 per
+// thread uncapped (2.92 against 2.63); whole-line writes (without them 2.0-2.2); the paired complex64
+// kernel under a 52 KB cap (5.9-6.2 TB/s on pairs from (4, 5) up, against 3.3 with 8 B accesses)
+constexpr PermShape kPerm = {1, 0, 4, 8, 40960, 0, 1, 1, 53248};
+static PermShape g_perm = kPerm;
+
+template <typename S, int U>
+void launch_perm_lane_u(hipStream_t st, void *psi, u64 dim, const PermLaneArgs &a, int xcd, int nt, int wl, int lds) {
+    const u64 per = (u64)SV_THREADS * U;
+    const unsigned grid = (unsigned)((dim + per - 1) / per);
+    S *p = (S *)psi;
+#define ORK_PERM_LAUNCH(X, N, W) sv_perm_lane_kernel<S, U, X, N, W><<<grid, SV_THREADS, lds, st>>>(p, dim, a)
+    if (xcd) {
+        if (nt) { if (wl) ORK_PERM_LAUNCH(1, 1, 1); else ORK_PERM_LAUNCH(1, 1, 0); }
+        else { if (wl) ORK_PERM_LAUNCH(1, 0, 1); else ORK_PERM_LAUNCH(1, 0, 0); }
+    } else {
+        if (nt) { if (wl) ORK_PERM_LAUNCH(0, 1, 1); else ORK_PERM_LAUNCH(0, 1, 0); }
+        else { if (wl) ORK_PERM_LAUNCH(0, 0, 1); else ORK_PERM_LAUNCH(0, 0, 0); }
+    }
+#undef ORK_PERM_LAUNCH
+}
+
+template <typename S>
+void launch_perm_lane(hipStream_t st, void *psi, u64 dim, const PermLaneArgs &a, int u, int xcd, int nt, int wl,
+                      int lds) {
+    if (u >= 8) launch_perm_lane_u<S, 8>(st, psi, dim, a, xcd, nt, wl, lds);
+    else if (u >= 4) launch_perm_lane_u<S, 4>(st, psi, dim, a, xcd, nt, wl, lds);
+    else launch_perm_lane_u<S, 2>(st, psi, dim, a, xcd, nt, wl, lds);
+}
+
+// flags: bits 0-1 log2 of the lane kernel's amplitudes per thread (1..3), bit 2 whole-line writes,
+// bit 3 lane kernel off, bit 4 lane kernel for every gate set below 6, bit 5 paired complex64 kernel
+// off, bit 6 the settings are for complex64 states (u, lds), else complex128; lds: residency cap of the
+// lane kernel (of the paired kernel when bit 7 is set)
+extern "C" int ork_sv_set_perm_shape(int flags, int lds) {
+    if (flags < 0 || flags > 255 || lds < 0 || lds > 160 * 1024 || (flags & 3) == 0) return (int)hipErrorInvalidValue;
+    const int u = 1 << (flags & 3);
+    g_perm.wl = (flags >> 2) & 1;
+    g_perm.lane_on = !((flags >> 3) & 1);
+    g_perm.lane_all = (flags >> 4) & 1;
+    g_perm.x2_on = !((flags >> 5) & 1);
+    if (flags & 128) {
+        g_perm.x2_lds = lds;
+    } else if (flags & 64) {
+        g_perm.lane_u32 = u;
+        g_perm.lane_lds32 = lds;
+    } else {
+        g_perm.lane_u = u;
+        g_perm.lane_lds = lds;
+    }
+    return 0;
+}
+
 extern "C" int ork_sv_set_mono_lds32(int lds) {
     if (lds < 0 || lds > 160 * 1024) return (int)hipErrorInvalidValue;
     g_mono_lds32 = lds;
@@ -716,9 +871,46 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
     const u64 nbase = ((u64)1 << n) >> m.k;
     hipStream_t st = (hipStream_t)stream;
     const int lds = prec == ORK_F32 ? g_mono_lds32 : 53248;   // residency cap (complex128: as the pair kernel)
-    // streaming accesses when no gate qubit lies inside a 128 B line (as for the pair kernel)
     const u64 elem = prec == ORK_F32 ? 8 : 16;
-    const bool nt = !(g_nt_off & 2) && (elem << m.pos[0]) >= 128 && (elem << n) >= g_nt_min_bytes;
+    const bool big = !(g_nt_off & 2) && (elem << n) >= g_nt_min_bytes;
+    const bool partial = (elem << m.pos[0]) < 128;   // a gate qubit inside a 128 B line
+    if (g_perm.lane_on && m.pos[m.k - 1] < 6 && (partial || g_perm.lane_all)) {
+        PermLaneArgs a;
+        a.k = mono->k;
+        for (int j = 0; j < 3; ++j) a.qs[j] = j < a.k ? mono->qs[j] : 0;
+        for (int x = 0; x < 8; ++x) { a.pinv[x] = x; a.ph[x] = 0; }
+        for (int x = 0; x < (1 << a.k); ++x) {
+            a.pinv[mono->perm[x]] = x;
+            a.ph[x] = mono->ph[x] & 3;
+        }
+        const u64 dim = (u64)1 << n;
+        const int u = prec == ORK_F32 ? g_perm.lane_u32 : g_perm.lane_u;
+        const int xcd = dim >= (u64)SV_THREADS * u * 8;
+        const int plds = prec == ORK_F32 ? g_perm.lane_lds32 : g_perm.lane_lds;
+        if (prec == ORK_F32) launch_perm_lane<float2>(st, psi, dim, a, u, xcd, big, g_perm.wl, plds);
+        else launch_perm_lane<double2>(st, psi, dim, a, u, xcd, big, g_perm.wl, plds);
+        ORK_LAUNCH_CHECK();
+        return 0;
+    }
+    // streaming accesses when no gate qubit lies inside a 128 B line (as for the pair kernel)
+    const bool nt = big && !partial;
+    if (prec == ORK_F32 && g_perm.x2_on && m.pos[0] >= 1) {
+        // pairs of amplitudes as 16 B elements: gate qubits one lower, offsets halved
+        MonoArgs m2 = m;
+        for (int i = 0; i < m.k; ++i) m2.pos[i] = m.pos[i] - 1;
+        for (int j = 0; j < m.nt; ++j) {
+            m2.src[j] = m.src[j] >> 1;
+            m2.dst[j] = m.dst[j] >> 1;
+        }
+        const u64 nb2 = nbase >> 1;
+        const bool nt2 = big && (16ull << m2.pos[0]) >= 128;
+        const u64 per = (u64)SV_THREADS * SV_UNROLL;
+        const unsigned grid = (unsigned)((nb2 + per - 1) / per);
+        if (nt2) sv_mono_x2_kernel<SV_UNROLL, 1><<<grid, SV_THREADS, g_perm.x2_lds, st>>>((f4v *)psi, nb2, m2);
+        else sv_mono_x2_kernel<SV_UNROLL, 0><<<grid, SV_THREADS, g_perm.x2_lds, st>>>((f4v *)psi, nb2, m2);
+        ORK_LAUNCH_CHECK();
+        return 0;
+    }
     if (prec == ORK_F32) {
         const u64 per = (u64)SV_THREADS * SV_UNROLL * 2;
         const unsigned grid = (unsigned)((nbase + per - 1) / per);
@@ -821,6 +1013,7 @@ extern "C" int ork_sv_set_nt_off(int mask, int min_bytes) {
 extern "C" unsigned long long ork_nt_min_bytes(void) { return g_nt_min_bytes; }
 
 extern "C" void ork_sv_reset_tuning(void) {
+    g_perm = kPerm;
     g_pair_override = SvShape{0, 0, 0, 0};
     g_nt_off = 0;
     g_nt_min_bytes = kNtMinBytes;

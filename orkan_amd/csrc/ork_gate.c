@@ -12,8 +12,10 @@
  *                cz, swap, ccx): src/gate/gate_pure.c:33-434
  *   mixed, diagonal gates (z, s, sdg, t, tdg, p, rz, cz): elementwise phase
  *                kernel, only the entries whose factor is not 1 are touched
- *   mixed, x / y / cx / cy / swap / ccx: tile-group kernel, permutation with
- *                phases applied during the store
+ *   mixed, x / y: streamed superoperator kernels (exact moves: entries 0, +-1, +-i)
+ *   mixed, cx / cy / swap / ccx: tile-group kernel, permutation with phases
+ *                applied during the store; entries the gate leaves unchanged are
+ *                neither read nor written
  *   mixed, h / hy / rx / ry / single_from_mat: tile-group kernel with the
  *                gate's 4x4 superoperator; its entries are the coefficients of
  *                the reference's closed forms (src/gate/packed/gate_packed_1q.c:205-594)
@@ -51,6 +53,28 @@ static void mono(state_t *s, int k, const int *qs, const int *perm, const int *p
     }
     if (ork_layout(s) == PURE) {
         ORK_HIP("gate", ork_sv_mono(stream(), ork_prec(s), s->data, s->qubits, &m));
+        return;
+    }
+    if (k == 1) {
+        /* x, y: the superoperator S[(i,j),(k,l)] = U[i][k] conj(U[j][l]) of U[perm a][a] = i^ph[a] has
+         * entries 0, +-1, +-i only, so the streamed superoperator kernels move every entry exactly */
+        static const double IRE[4] = {1.0, 0.0, -1.0, 0.0}, IIM[4] = {0.0, 1.0, 0.0, -1.0};
+        double ure[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, uim[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        for (int a = 0; a < 2; ++a) {
+            ure[perm[a]][a] = IRE[ph[a] & 3];
+            uim[perm[a]][a] = IIM[ph[a] & 3];
+        }
+        double S[32];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int kk = 0; kk < 2; ++kk)
+                    for (int l = 0; l < 2; ++l) {
+                        /* U[i][kk] * conj(U[j][l]) */
+                        const double ar = ure[i][kk], ai = uim[i][kk], br = ure[j][l], bi = -uim[j][l];
+                        S[2 * ((2 * i + j) * 4 + 2 * kk + l)] = ar * br - ai * bi;
+                        S[2 * ((2 * i + j) * 4 + 2 * kk + l) + 1] = ar * bi + ai * br;
+                    }
+        ORK_HIP("gate", ork_dm_group(stream(), ork_prec(s), ork_layout(s), L_of(), s->data, s->qubits, ORK_DM_SOP, &m, S));
         return;
     }
     ORK_HIP("gate", ork_dm_group(stream(), ork_prec(s), ork_layout(s), L_of(), s->data, s->qubits, ORK_DM_MONO, &m, NULL));

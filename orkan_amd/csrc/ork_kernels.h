@@ -99,6 +99,7 @@ int ork_dm_set_shape(int which, int flags, int lds_bytes);
 int ork_sv_set_red_lds(int flags, int lds_bytes);
 int ork_sv_set_exp_shape(int flags, int lds_bytes);
 int ork_sv_set_mono_lds32(int lds_bytes);
+int ork_sv_set_perm_shape(int flags, int lds_bytes);
 int ork_mlp_gemm_select(int variant);
 int ork_sv_set_pair_shape(int flags, int lds_bytes);
 int ork_sv_set_nt_off(int mask, int min_bytes);

@@ -114,6 +114,7 @@ int orkan_set_launch_shape(int kernel, int flags, int lds_bytes) {
     if (kernel == 17) return ork_sv_set_layer_variant(flags);
     if (kernel == 19) return ork_sv_set_exp_shape(flags, lds_bytes);
     if (kernel == 20) return ork_sv_set_mono_lds32(lds_bytes);
+    if (kernel == 21) return ork_sv_set_perm_shape(flags, lds_bytes);
     if (kernel == 14) {
         ork_dm_reset_shapes();
         ork_sv_reset_tuning();

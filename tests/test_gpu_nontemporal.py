@@ -104,17 +104,20 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, eng3, lt, kind, 
     S = wl.superop(wl.kraus_amp_damp(0.05))
     diag = wl.rng(44).uniform(-1, 1, 1 << n)
 
-    def run(nt):
+    def run(mode):
+        # mode 0: cached accesses; 1: nontemporal; 2: nontemporal with bit 6 (packed superoperator: only
+        # on the lines a workgroup owns alone; phase kernels: whole-line writes)
+        nt, b6 = min(mode, 1), 64 * (mode == 2)
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(NT_OFF_HOOK, 0, 1) == 0
         for k6, k7 in ((6, 7), (15, 16)):   # complex128 and complex64 shapes
             # with nt: the lane-exchange kernel (bit 3) for in-tile targets whose quad columns share lines
             assert api.orkan_set_launch_shape(k6, 1 | 12 * nt, 65536) == 0
             assert api.orkan_set_launch_shape(k7, 4 * nt, 81920) == 0
-        assert api.orkan_set_launch_shape(9, 3 | 4 * nt, 81920) == 0
-        assert api.orkan_set_launch_shape(18, 3 | 4 * nt, 81920) == 0
-        assert api.orkan_set_launch_shape(4, 3 | 4 * nt, 0) == 0
-        assert api.orkan_set_launch_shape(5, 4 * nt, 0) == 0
+        assert api.orkan_set_launch_shape(9, 3 | 4 * nt | b6, 81920) == 0
+        assert api.orkan_set_launch_shape(18, 3 | 4 * nt | b6, 81920) == 0
+        assert api.orkan_set_launch_shape(4, 3 | 4 * nt | b6, 0) == 0
+        assert api.orkan_set_launch_shape(5, 4 * nt | b6, 0) == 0
         st = eng.new(kind, n, rho)
         if fp32:
             st32 = api.orkan_state_convert(C.byref(st), 1)
@@ -136,7 +139,9 @@ def test_density_matrix_streaming_accesses_change_no_value(eng, eng3, lt, kind, 
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         return out
 
-    assert np.array_equal(run(1), run(0))
+    base = run(0)
+    assert np.array_equal(run(1), base)
+    assert np.array_equal(run(2), base)
 
 
 @pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED])

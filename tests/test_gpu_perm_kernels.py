@@ -1,0 +1,174 @@
+"""Permutation gates move amplitudes exactly, so every kernel that can run one gives bit-identical
+results.  Statevector: the lane-exchange kernel (every gate qubit below 6), the paired complex64
+kernel (no gate qubit at bit 0, two amplitudes per 16 B access) and the per-group kernel, under every
+launch shape of tuning hook 21, with nontemporal accesses forced on through hook 13.  Density matrix:
+x and y run through the streamed superoperator kernels (their superoperators hold 0, +-1, +-i only)
+and cx / cy / swap / ccx through the tile-group kernel, which leaves unchanged entries unread and
+unwritten; both are checked against the oracle on every position, and the streamed x / y against
+the tile-group kernel bit for bit (hook 8 switches the streamed kernels off)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import engines
+import orkan_amd
+from linalg_np import MIXED_PACKED, MIXED_TILED, PURE, random_psi, random_rho, to_layout
+
+pytestmark = pytest.mark.gpu
+NT_HOOK, RESET_HOOK, STREAM_HOOK, PERM_HOOK, PERMT_HOOK = 13, 14, 8, 21, 22
+GROUP_PACKED_HOOK, GROUP_TILED_HOOK = 2, 3
+
+GATES = [("cx", (0, 1)), ("cx", (1, 0)), ("cx", (1, 2)), ("cx", (2, 3)), ("cx", (3, 4)), ("cx", (5, 0)),
+         ("cx", (0, 9)), ("cx", (9, 10)), ("cy", (0, 1)), ("cy", (2, 1)), ("cy", (4, 7)), ("swap_gate", (0, 1)),
+         ("swap_gate", (1, 5)), ("swap_gate", (6, 8)), ("ccx", (0, 1, 2)), ("ccx", (2, 0, 1)), ("ccx", (1, 3, 4)),
+         ("ccx", (5, 9, 2)), ("ccx", (7, 8, 9))]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = engines.DeviceEngine(orkan_amd.lib_path(5), 5)
+    yield e
+    assert e.api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+
+
+def _sv(eng, n, psi, fp32):
+    api = eng.api
+    st = eng.new(PURE, n, psi)
+    if fp32:
+        s32 = api.orkan_state_convert(C.byref(st), 1)
+        eng.free(st)
+        st = s32
+    for name, qs in GATES:
+        getattr(api, name)(C.byref(st), *qs)
+    out = np.empty(1 << n, np.complex64 if fp32 else np.complex128)
+    assert api.orkan_state_download(C.byref(st), out.ctypes.data) == 0
+    eng.free(st)
+    return out
+
+
+@pytest.mark.parametrize("fp32", [False, True], ids=["c128", "c64"])
+def test_sv_permutation_kernels_bit_identical(eng, fp32):
+    api = eng.api
+    n = 12
+    psi = random_psi(n, np.random.default_rng(31))
+    f32 = 64 if fp32 else 0
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    # the per-group kernels only: lane kernel and paired kernel off
+    assert api.orkan_set_launch_shape(PERM_HOOK, 1 | 8 | 32 | f32, 0) == 0
+    want = _sv(eng, n, psi, fp32)
+    # the oracle agrees with the per-group kernels
+    orc = engines.oracle(5)
+    so = orc.new(PURE, n, psi)
+    for name, qs in GATES:
+        orc.gate(so, name, *qs)
+    ref = orc.read(so)
+    orc.free(so)
+    assert float(np.abs(want.astype(np.complex128) - (ref.astype(np.complex64) if fp32 else ref)).max()) <= 1e-12
+    shapes = []
+    for lu in (1, 2, 3):
+        for wl in (0, 4):
+            for extra in (0, 16):
+                shapes.append((lu | wl | extra | f32, 0))
+    shapes += [(2 | 4 | f32, 40960), (2 | 4 | 8 | f32, 0)]
+    if fp32:
+        shapes += [(3 | 4 | 64 | 128, 26624), (3 | 4 | 64 | 32, 0)]
+    for nt_min in (0, 1):   # 1: nontemporal accesses on every state size
+        assert api.orkan_set_launch_shape(NT_HOOK, 0, nt_min) == 0
+        for flags, lds in shapes:
+            assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+            assert api.orkan_set_launch_shape(NT_HOOK, 0, nt_min) == 0
+            assert api.orkan_set_launch_shape(PERM_HOOK, flags, lds) == 0
+            got = _sv(eng, n, psi, fp32)
+            assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (flags, lds, nt_min)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+@pytest.mark.parametrize("log_tile", [5, 3])
+def test_dm_permutations_against_oracle(kind, log_tile):
+    """Every permutation gate on many positions against the oracle, with the tile-group kernel's loads
+    one after the other (the default) and all in flight at once (group shape flag bit 6): the two
+    forms are bit-identical."""
+    n = 8
+    dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
+    api = dev.api
+    orc = engines.oracle(log_tile)
+    rng = np.random.default_rng(77 + kind + log_tile)
+    rho = to_layout(random_rho(n, rng), kind, log_tile)
+    ops = [("x", (q,)) for q in range(n)] + [("y", (q,)) for q in range(n)]
+    ops += [(g, (a, b)) for g in ("cx", "cy", "swap_gate") for a in range(n) for b in range(n) if a != b and (a + 2 * b) % 3 == 0]
+    ops += [("ccx", (a, b, c)) for (a, b, c) in ((0, 1, 2), (2, 1, 0), (7, 0, 4), (3, 6, 5), (5, 7, 6), (1, 4, 7))]
+    so = orc.new(kind, n, rho)
+    for name, qs in ops:
+        orc.gate(so, name, *qs)
+    want = orc.read(so)
+    orc.free(so)
+    outs = []
+    for hoist in (0, 64):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        if kind == MIXED_PACKED:
+            assert api.orkan_set_launch_shape(GROUP_PACKED_HOOK, 1 | 2 | hoist, 0) == 0
+        else:
+            assert api.orkan_set_launch_shape(GROUP_TILED_HOOK, 1 | hoist, 0) == 0
+        sd = dev.new(kind, n, rho)
+        for name, qs in ops:
+            dev.gate(sd, name, *qs)
+        outs.append(dev.read(sd))
+        dev.free(sd)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert float(np.abs(outs[0] - want).max()) <= 1e-12
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+def test_dm_x_y_streamed_equal_tile_groups(eng, kind):
+    api = eng.api
+    n = 9
+    rho = to_layout(random_rho(n, np.random.default_rng(5)), kind, 5)
+    outs = []
+    for streamed in (1, 0):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(STREAM_HOOK, 0, streamed) == 0
+        st = eng.new(kind, n, rho)
+        for q in range(n):
+            api.x(C.byref(st), q)
+            api.y(C.byref(st), (q + 3) % n)
+        outs.append(eng.read(st))
+        eng.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    # equal entry by entry (the superoperator's zero products may only flip the sign of a zero)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+@pytest.mark.parametrize("log_tile", [5, 3])
+def test_dm_tile_permutations_equal_tile_groups(kind, log_tile):
+    """Gates whose qubits all lie at or above the tile (32 x 32 blocks for packed) move whole tiles
+    (dm_perm_tiles_kernel); hook 22 with flag bit 1 sends them to the tile-group kernel instead.  Both
+    move entries exactly: bit-identical, and equal to the oracle."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
+    api = dev.api
+    orc = engines.oracle(log_tile)
+    T = log_tile if kind == MIXED_TILED else 5
+    n = T + 3
+    hi = list(range(T, n))
+    ops = [(g, (a, b)) for g in ("cx", "cy", "swap_gate") for a in hi for b in hi if a != b]
+    ops += [("ccx", (a, b, c)) for a in hi for b in hi for c in hi if len({a, b, c}) == 3]
+    rho = to_layout(random_rho(n, np.random.default_rng(90 + kind + log_tile)), kind, log_tile)
+    outs = []
+    for off in (0, 2):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | off, 0) == 0
+        st = dev.new(kind, n, rho)
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        outs.append(dev.read(st))
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert np.array_equal(outs[0], outs[1])
+    so = orc.new(kind, n, rho)
+    for name, qs in ops:
+        orc.gate(so, name, *qs)
+    assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
+    orc.free(so)
