@@ -149,7 +149,8 @@ __device__ __forceinline__ cd lds_get(const cd *lds, const DmOp &op, int x, int 
 
 // HOIST: every load of a thread in flight before the first LDS write (4 per thread, at the cost of
 // registers), else one load after the other at the occupancy the smaller register count allows
-template <typename S, int LAYOUT, int OP, int HOIST>
+// NTA: nontemporal loads and stores (tiled layout with virtual tiles of whole 128 B rows)
+template <typename S, int LAYOUT, int OP, int HOIST, int NTA>
 __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rho, DmPlan p, DmOp op) {
     __shared__ cd lds[W_MAX * LDS_STRIDE];
     constexpr int EPT = W_MAX * W_MAX / DM_THREADS;   // entries of a group per thread
@@ -189,14 +190,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                     ly[u] = j * B + sc;
                     if (!moved<OP>(op, lx[u], ly[u])) continue;
                     lk[u] = (LAYOUT == ORK_PACKED && TR == TC && sr > sc) ? 2 : 1;
-                    lv[u] = ldc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
+                    lv[u] = ldx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
                 } else {
                     // stored mirror tile (TC, TR): its entry (sr, sc) is conj of ours at (sc, sr)
                     lx[u] = i * B + sc;
                     ly[u] = j * B + sr;
                     if (!moved<OP>(op, lx[u], ly[u])) continue;
                     lk[u] = 3;
-                    lv[u] = ldc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
+                    lv[u] = ldx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
                 }
             }
 #pragma unroll
@@ -216,13 +217,13 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 if (tile_stored<LAYOUT>(p, TR, TC)) {
                     if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
                     if (!moved<OP>(op, i * B + sr, j * B + sc)) continue;
-                    const cd v = ldc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
+                    const cd v = ldx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
                     lds[(i * B + sr) * LDS_STRIDE + j * B + sc] = v;
                     if (LAYOUT == ORK_PACKED && TR == TC && sr > sc)
                         lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
                 } else {
                     if (!moved<OP>(op, i * B + sc, j * B + sr)) continue;
-                    const cd v = ldc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
+                    const cd v = ldx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
                     lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
                 }
             }
@@ -337,8 +338,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
             const bool direct = tile_stored<LAYOUT>(p, TR, TC) && !(LAYOUT == ORK_PACKED && TR == TC && sr < sc) &&
                                 moved<OP>(op, i * B + sr, j * B + sc);
             const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR) && moved<OP>(op, i * B + sc, j * B + sr);
-            if (direct) stc(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, op, i * B + sr, j * B + sc));
-            if (mirror) stc(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, op, i * B + sc, j * B + sr)));
+            if (direct) stx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, op, i * B + sr, j * B + sc));
+            if (mirror) stx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, op, i * B + sc, j * B + sr)));
         }
         __syncthreads();
     }
@@ -810,24 +811,22 @@ __device__ __forceinline__ void perm_tile_entry(const PermTilePlan &p, u64 TR, u
     }
 }
 
-// TRANS 0: the tile pairs whose source lies below the diagonal (and tiles that only change phase), with
-// no LDS; PARTS consecutive workgroups share a pair, each taking 1 / PARTS of both tiles in memory order
-// (the same positions in s and p).  TRANS 1: the pairs whose source is a conjugate transpose, whole in
-// one workgroup through LDS (a split would let one workgroup read entries another has rewritten).
-template <typename S, int LAYOUT, int PARTS, int TRANS>
+// One workgroup per stored tile s; pairs whose source is a conjugate transpose go through one LDS tile
+// used twice (stage p, write s; stage s, write p), the whole pair in one workgroup: split, one
+// workgroup would read entries another had rewritten.  (A second launch for the transposed pairs alone
+// spent most of its time dispatching the workgroups of the other tiles.)
+template <typename S, int LAYOUT, int NTA>
 __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict__ rho, PermTilePlan p) {
-    constexpr int E = W_MAX * W_MAX / DM_THREADS / PARTS;   // entries of each tile per thread
+    __shared__ cd ts[W_MAX * LDS_STRIDE];
+    constexpr int E = W_MAX * W_MAX / DM_THREADS;   // entries of each tile per thread
     const int T = LAYOUT == ORK_TILED ? (1 << p.L) : 32, TT = T * T, tid = threadIdx.x;
-    const u64 g = remap_group(blockIdx.x, p.ntiles * PARTS, p.xcd);
-    const u64 w = g / PARTS;
-    const int part = (int)(g % PARTS);
+    const u64 w = remap_group(blockIdx.x, p.ntiles, p.xcd);
     if (w >= p.ntiles) return;
     u64 TR, TC;
     tri_decode(w, TR, TC);
     int sa, sb;
     const u64 PR = perm_src_tile(p, TR, sa), PC = perm_src_tile(p, TC, sb);
     const bool trans = PR < PC;
-    if (trans != (bool)TRANS) return;
     const u64 pr = trans ? PC : PR, pc = trans ? PR : PC;
     const u64 wp = pr * (pr + 1) / 2 + pc;
     if (wp < w) return;   // the workgroup of p swaps both
@@ -838,18 +837,17 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
     (void)perm_src_tile(p, pc, pb);
     const int php = (p.ph[pa] - p.ph[pb]) & 3;
     const bool pair = wp != w;
-    const int o0 = part * (TT / PARTS);
     cd vs[E], vp[E];
     u64 os[E], op[E];
     bool ok[E];
     int ii[E], jj[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int o = o0 + tid + e * DM_THREADS;
+        const int o = tid + e * DM_THREADS;
         bool st = false;
         int i = 0, j = 0;
         u64 off = 0, offp = 0;
-        if (o < o0 + TT / PARTS && o < TT) perm_tile_entry<LAYOUT>(p, TR, TC, o, i, j, off, st);
+        if (o < TT) perm_tile_entry<LAYOUT>(p, TR, TC, o, i, j, off, st);
         ok[e] = st;
         ii[e] = i;
         jj[e] = j;
@@ -860,35 +858,102 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
             perm_tile_entry<LAYOUT>(p, pr, pc, o, i2, j2, offp, stp);
         }
         op[e] = offp;
-        vs[e] = st ? ldc(rho + off) : cmk(0.0, 0.0);
-        vp[e] = (st && pair) ? ldc(rho + offp) : cmk(0.0, 0.0);
+        vs[e] = st ? ldx<NTA>(rho + off) : cmk(0.0, 0.0);
+        vp[e] = (st && pair) ? ldx<NTA>(rho + offp) : cmk(0.0, 0.0);
     }
-    if constexpr (!TRANS) {
+    if (!trans) {
         // same in-tile positions: s takes p's entries and p takes s's (or s its own, phase only)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             if (!ok[e]) continue;
-            stc(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
-            if (pair) stc(rho + op[e], cipow(vs[e], php));
+            stx<NTA>(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
+            if (pair) stx<NTA>(rho + op[e], cipow(vs[e], php));
         }
-    } else {
-        // new s (i, j) = i^phs conj(old p (j, i)), new p (i, j) = i^php conj(old s (j, i)); off the
-        // diagonal every entry of both tiles is stored
-        __shared__ cd ts[W_MAX * LDS_STRIDE];
-        __shared__ cd tp[W_MAX * LDS_STRIDE];
+        return;
+    }
+    // new s (i, j) = i^phs conj(old p (j, i)), new p (i, j) = i^php conj(old s (j, i)); off the diagonal
+    // every entry of both tiles is stored, and every old value is already in registers
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (!ok[e]) continue;
-            ts[ii[e] * LDS_STRIDE + jj[e]] = vs[e];
-            tp[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
-        }
-        __syncthreads();
+    for (int e = 0; e < E; ++e)
+        if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
+    __syncthreads();
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (!ok[e]) continue;
-            stc(rho + os[e], cipow(cconj(tp[jj[e] * LDS_STRIDE + ii[e]]), phs));
-            if (pair) stc(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
+    for (int e = 0; e < E; ++e)
+        if (ok[e]) stx<NTA>(rho + os[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), phs));
+    if (!pair) return;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = vs[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (ok[e]) stx<NTA>(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
+}
+
+// ---- permutations inside 32 x 32 tiles: every gate qubit below 5 -------------------------------
+// When every qubit of cx, cy, swap or ccx lies below 5 (32 x 32 storage tiles; packed: 32 x 32 blocks),
+// each tile maps onto itself: new (i, j) = i^(pha[i] - pha[j]) old (P i, P j) with P and pha 32-entry
+// tables of the in-tile index (P: the source's in-tile index, pha: the phase exponent of the source's
+// gate bits).  One workgroup per stored tile: its changed entries (the sources of changed entries are
+// changed entries) are loaded with every load of a thread in flight, staged in LDS (a packed diagonal
+// block fills its upper half from the conjugated lower half), and stored from the table lookup.
+struct PermInPlan {
+    u64 ntiles, D;
+    int L;              // tiled: 5; packed: 5 (blocks)
+    int xcd;
+    signed char P[32];  // in-tile index -> source in-tile index
+    signed char ph[32]; // in-tile index -> phase exponent (mod 4) of its source
+};
+
+template <typename S, int LAYOUT, int NTA>
+__global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restrict__ rho, PermInPlan p) {
+    __shared__ cd t[32 * LDS_STRIDE];
+    const int tid = threadIdx.x;
+    const u64 w = remap_group(blockIdx.x, p.ntiles, p.xcd);
+    if (w >= p.ntiles) return;
+    u64 TR, TC;
+    tri_decode(w, TR, TC);
+    const bool dg = TR == TC;
+    cd v[4];
+    u64 off[4];
+    bool ld[4];
+    int ii[4], jj[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int o = tid + e * DM_THREADS;
+        int i, j;
+        if constexpr (LAYOUT == ORK_TILED) {
+            i = o >> 5;
+            j = o & 31;
+            off[e] = tile_base(TR, TC, 5) + (u64)o;
+        } else {
+            j = o >> 5;
+            i = o & 31;
+            const u64 r = TR * 32 + (u64)i, c = TC * 32 + (u64)j;
+            off[e] = r >= c ? col_base(p.D, c) + (r - c) : 0;
         }
+        ii[e] = i;
+        jj[e] = j;
+        const bool stored = LAYOUT == ORK_TILED || !dg || i >= j;
+        // changed: moves or takes a phase (in a packed diagonal block, an entry is also needed as the
+        // mirror of its upper-half partner, which changes exactly when it does)
+        const bool chg = p.P[i] != i || p.P[j] != j || ((p.ph[i] - p.ph[j]) & 3);
+        ld[e] = stored && chg;
+        v[e] = ld[e] ? ldx<NTA>(rho + off[e]) : cmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (!ld[e]) continue;
+        t[ii[e] * LDS_STRIDE + jj[e]] = v[e];
+        if (LAYOUT == ORK_PACKED && dg && ii[e] > jj[e]) t[jj[e] * LDS_STRIDE + ii[e]] = cconj(v[e]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (!ld[e]) continue;
+        const int i = ii[e], j = jj[e];
+        stx<NTA>(rho + off[e], cipow(t[p.P[i] * LDS_STRIDE + p.P[j]], p.ph[i] - p.ph[j]));
     }
 }
 
@@ -1103,7 +1168,7 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 // nt: nontemporal loads and stores; lp: log2 of the workgroups sharing one tile quad (cross-tile kernel)
 // wl: whole-line writes of the phase kernels (flags bit 6)
 struct Shape { int xcd, col, lds, nt, lp, wl; };
-constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 0}};   // [packed, tiled]
+constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 1}};   // [packed, tiled]
 // phase kernels (cz, exp_diag, ...): flat streams of E entries per thread, XCD runs of 32 workgroups,
 // nontemporal accesses, whole-line writes where the changed entries do not fill lines
 constexpr Shape kPhaseShape[2] = {{2, 0, 0, 1, 0, 1}, {2, 0, 0, 1, 0, 1}};
@@ -1122,11 +1187,21 @@ int launch_group(hipStream_t st, void *rho, DmPlan p, int opk, const DmOp &op) {
     p.xcd = sh.xcd;
     p.col = sh.col;
     const int lds = sh.lds > (int)sizeof(cd) * W_MAX * LDS_STRIDE ? sh.lds - (int)sizeof(cd) * W_MAX * LDS_STRIDE : 0;
-    // bit 6 of the group shape flags (the wl field): every load of a thread in flight at once
-#define ORK_GROUP_LAUNCH(OPC)                                                                            \
-    do {                                                                                                 \
-        if (sh.wl) dm_group_kernel<S, LAYOUT, OPC, 1><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op); \
-        else dm_group_kernel<S, LAYOUT, OPC, 0><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);       \
+    // bit 6 of the group shape flags (the wl field): every load of a thread in flight at once; bit 2
+    // (nt): nontemporal accesses on the tiled layout where virtual tiles are whole 128 B rows, on states
+    // of at least the nontemporal minimum
+    const u64 tl = p.dim >= ((u64)1 << p.L) ? p.dim >> p.L : 1;
+    const bool nt = LAYOUT == ORK_TILED && sh.nt && ((u64)sizeof(S) << p.LB) >= 128 &&
+                    ((tl * (tl + 1) / 2) << (2 * p.L)) * sizeof(S) >= ork_nt_min_bytes();
+#define ORK_GROUP_LAUNCH(OPC)                                                                                    \
+    do {                                                                                                         \
+        if (nt) {                                                                                                \
+            if (sh.wl) dm_group_kernel<S, LAYOUT, OPC, 1, 1><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);   \
+            else dm_group_kernel<S, LAYOUT, OPC, 0, 1><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);         \
+        } else {                                                                                                 \
+            if (sh.wl) dm_group_kernel<S, LAYOUT, OPC, 1, 0><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);   \
+            else dm_group_kernel<S, LAYOUT, OPC, 0, 0><<<grid, DM_THREADS, lds, st>>>((S *)rho, p, op);         \
+        }                                                                                                        \
     } while (0)
     switch (opk) {
     case ORK_DM_SOP: ORK_GROUP_LAUNCH(ORK_DM_SOP); break;
@@ -1265,9 +1340,9 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
 }
 
 // whole-tile permutations: XCD runs of 32 workgroups, one workgroup per tile pair (splitting a pair over 2
-// or 4 lost 4-25 % on 14 qubits); `col` set switches the kernel off (tile groups instead); set through
-// orkan_set_launch_shape 22
-constexpr Shape kPermTileShape = {2, 0, 0, 0, 0, 0};
+// or 4 workgroups lost 4-25 % on 14 qubits), nontemporal accesses on the tiled layout; `col` set switches
+// the kernel off (tile groups instead); set through orkan_set_launch_shape 22
+constexpr Shape kPermTileShape = {2, 0, 0, 1, 0, 0};
 Shape g_permt_shape = kPermTileShape;
 
 // can the gate run through dm_perm_tiles_kernel?  An involution whose qubits all lie at or above the
@@ -1281,6 +1356,47 @@ bool perm_tiles_ok(int layout, int L, int n, const ork_mono_t *m) {
     for (int a = 0; a < (1 << m->k); ++a)
         if (m->perm[m->perm[a]] != a) return false;
     return true;
+}
+
+// can the gate run through dm_perm_intile_kernel?  Every qubit below 5, 32 x 32 tiles (tiled with
+// LOG_TILE_DIM 5, or packed blocks) and at least one whole tile
+bool perm_intile_ok(int layout, int L, int n, const ork_mono_t *m) {
+    if (g_permt_shape.col || n < 5 || (layout == ORK_TILED && L != 5)) return false;
+    for (int j = 0; j < m->k; ++j)
+        if (m->qs[j] >= 5) return false;
+    return true;
+}
+
+template <typename S, int LAYOUT>
+int launch_perm_intile(hipStream_t st, void *rho, int n, const ork_mono_t *m) {
+    PermInPlan p;
+    p.D = (u64)1 << n;
+    p.L = 5;
+    const u64 nt = p.D >> 5;
+    p.ntiles = nt * (nt + 1) / 2;
+    if (p.ntiles > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    int pinv[8];
+    for (int a = 0; a < 8; ++a) pinv[a] = a;
+    for (int a = 0; a < (1 << m->k); ++a) pinv[m->perm[a]] = a;
+    for (int x = 0; x < 32; ++x) {
+        int a = 0;
+        for (int j = 0; j < m->k; ++j) a |= ((x >> m->qs[j]) & 1) << j;
+        const int sa = pinv[a];
+        int y = x;
+        for (int j = 0; j < m->k; ++j)
+            if (((a ^ sa) >> j) & 1) y ^= 1 << m->qs[j];
+        p.P[x] = (signed char)y;
+        p.ph[x] = (signed char)(m->ph[sa] & 3);
+    }
+    const Shape sh = g_permt_shape;
+    p.xcd = sh.xcd;
+    const bool ntm = LAYOUT == ORK_TILED && sh.nt && (p.ntiles << 10) * sizeof(S) >= ork_nt_min_bytes();
+    const int stat = (int)sizeof(cd) * 32 * LDS_STRIDE;
+    const int lds = sh.lds > stat ? sh.lds - stat : 0;
+    if (ntm) dm_perm_intile_kernel<S, LAYOUT, 1><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>((S *)rho, p);
+    else dm_perm_intile_kernel<S, LAYOUT, 0><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>((S *)rho, p);
+    ORK_LAUNCH_CHECK();
+    return 0;
 }
 
 template <typename S, int LAYOUT>
@@ -1300,16 +1416,15 @@ int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t 
     }
     const Shape sh = g_permt_shape;
     p.xcd = sh.xcd;
-    // direct pairs split over 1 << lp workgroups (32 x 32 tiles only), then the transposed pairs
-    const int lp = (LAYOUT == ORK_PACKED || L == 5) ? sh.lp : 0;
-    const unsigned g0 = (unsigned)(p.ntiles << lp);
-    if ((p.ntiles << lp) > 0x7fffffffu) return (int)hipErrorInvalidValue;
     S *r = (S *)rho;
-    if (lp == 2) dm_perm_tiles_kernel<S, LAYOUT, 4, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
-    else if (lp == 1) dm_perm_tiles_kernel<S, LAYOUT, 2, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
-    else dm_perm_tiles_kernel<S, LAYOUT, 1, 0><<<g0, DM_THREADS, sh.lds, st>>>(r, p);
-    ORK_LAUNCH_CHECK();
-    dm_perm_tiles_kernel<S, LAYOUT, 1, 1><<<(unsigned)p.ntiles, DM_THREADS, 0, st>>>(r, p);
+    // nontemporal accesses (flags bit 2) on the tiled layout, whose tiles are whole lines, on states of at
+    // least the nontemporal minimum; the packed layout's block columns share lines with their neighbours
+    const bool ntm = LAYOUT == ORK_TILED && sh.nt && (p.ntiles << (2 * p.L)) * sizeof(S) >= ork_nt_min_bytes() &&
+                     ((u64)sizeof(S) << p.L) >= 128;
+    const int stat = (int)sizeof(cd) * W_MAX * LDS_STRIDE;
+    const int lds = sh.lds > stat ? sh.lds - stat : 0;
+    if (ntm) dm_perm_tiles_kernel<S, LAYOUT, 1><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>(r, p);
+    else dm_perm_tiles_kernel<S, LAYOUT, 0><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>(r, p);
     ORK_LAUNCH_CHECK();
     return 0;
 }
@@ -1503,6 +1618,13 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     op.s2 = opk == ORK_DM_SOP2 ? (const cd *)mat : nullptr;
     if (opk == ORK_DM_SOP2 && k != 2) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
+    if (opk == ORK_DM_MONO && perm_intile_ok(layout, log_tile, n, m)) {
+        if (prec == ORK_F32)
+            return layout == ORK_PACKED ? launch_perm_intile<float2, ORK_PACKED>(st, rho, n, m)
+                                        : launch_perm_intile<float2, ORK_TILED>(st, rho, n, m);
+        return layout == ORK_PACKED ? launch_perm_intile<double2, ORK_PACKED>(st, rho, n, m)
+                                    : launch_perm_intile<double2, ORK_TILED>(st, rho, n, m);
+    }
     if (opk == ORK_DM_MONO && perm_tiles_ok(layout, log_tile, n, m)) {
         if (prec == ORK_F32)
             return layout == ORK_PACKED ? launch_perm_tiles<float2, ORK_PACKED>(st, rho, n, log_tile, m)

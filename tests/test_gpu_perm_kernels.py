@@ -172,3 +172,66 @@ def test_dm_tile_permutations_equal_tile_groups(kind, log_tile):
         orc.gate(so, name, *qs)
     assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
     orc.free(so)
+
+
+@pytest.mark.parametrize("log_tile", [5, 3])
+def test_dm_tiled_streaming_accesses_change_no_value(log_tile):
+    """Nontemporal accesses of the tile-group kernel (tiled layout, virtual tiles of whole lines) and of
+    the whole-tile permutations change the cache policy only: with hook 13 lowering the size limit to one
+    byte they are used on a 9-qubit state, and the results equal those with them switched off (bit 2 of
+    the kernel 3 and 22 shapes) bit for bit."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
+    api = dev.api
+    n = 9
+    rng = np.random.default_rng(123 + log_tile)
+    rho = to_layout(random_rho(n, rng), MIXED_TILED, log_tile)
+    U = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+    Um = np.ascontiguousarray(U.T.reshape(-1), np.complex128)   # column-major, as two_from_mat takes it
+    ops = [("cx", (0, 1)), ("cx", (7, 8)), ("ccx", (6, 7, 8)), ("swap_gate", (2, 7)), ("cy", (8, 3)),
+           ("ccx", (0, 5, 8)), ("swap_gate", (5, 6))]
+    outs = []
+    for nt in (4, 0):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(NT_HOOK, 0, 1) == 0
+        assert api.orkan_set_launch_shape(GROUP_TILED_HOOK, 1 | nt, 0) == 0
+        assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | nt, 0) == 0
+        st = dev.new(MIXED_TILED, n, rho)
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        for q1, q2 in ((0, 6), (7, 2), (4, 5)):
+            api.two_from_mat(C.byref(st), q1, q2, Um.ctypes.data)
+        outs.append(dev.read(st))
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+@pytest.mark.parametrize("n", [5, 7])
+def test_dm_intile_permutations_equal_tile_groups(kind, n):
+    """Gates whose qubits all lie below 5 permute each 32 x 32 tile (packed: block) within itself
+    (dm_perm_intile_kernel); hook 22 with flag bit 1 sends them to the tile-group kernel instead.  Both
+    move entries exactly: bit-identical, and equal to the oracle (n = 5 is a single tile)."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(5), 5)
+    api = dev.api
+    orc = engines.oracle(5)
+    lo = list(range(5))
+    ops = [(g, (a, b)) for g in ("cx", "cy", "swap_gate") for a in lo for b in lo if a != b]
+    ops += [("ccx", (a, b, c)) for a in lo for b in lo for c in lo if len({a, b, c}) == 3 and (a + b + c) % 2 == 0]
+    rho = to_layout(random_rho(n, np.random.default_rng(200 + kind + n)), kind, 5)
+    outs = []
+    for off in (0, 2):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | 4 | off, 0) == 0
+        st = dev.new(kind, n, rho)
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        outs.append(dev.read(st))
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert np.array_equal(outs[0], outs[1])
+    so = orc.new(kind, n, rho)
+    for name, qs in ops:
+        orc.gate(so, name, *qs)
+    assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
+    orc.free(so)
