@@ -6,9 +6,10 @@
 //   mean(C)
 // (src/state/state_pure.c:30-43, src/circ/circ_pure.c:21-34,
 //  src/gate/gate_pure.c:221-234, src/meas/meas_pure.c:20-30), with the same
-// per-amplitude formulas, so each instance matches a per-instance run of those
-// calls to rounding (the rx of one layer act on distinct qubits and commute;
-// states in HBM apply them in another order than 0..n-1).
+// per-amplitude formulas for the phases and the rx with their common factor
+// taken out (restored in the mean), so each instance matches a per-instance run
+// of those calls to rounding (the rx of one layer act on distinct qubits and
+// commute; states in HBM apply them in another order than 0..n-1).
 //
 // Execution: every workgroup of the persistent grid pulls evaluations from an
 // atomic counter (largest first) and runs each one start to finish, so no
@@ -50,12 +51,33 @@ struct QaoaArgs {
     unsigned *counter;
 };
 
-// rx(2 beta) on the pair (a, b): a' = c a - i s b, b' = -i s a + c b (gate_pure.c:221-234)
-__device__ __forceinline__ void rx_pair(cd &a, cd &b, double c, double s) {
-    const cd na = cmk(c * a.x + s * b.y, c * a.y - s * b.x);
-    const cd nb = cmk(s * a.y + c * b.x, -s * a.x + c * b.y);
-    a = na;
-    b = nb;
+// rx(2 beta) on the pair (a, b): a' = c a - i s b, b' = -i s a + c b (gate_pure.c:221-234), with its
+// scalar factor taken out: for |c| >= |s|, rx = c (I - i u X) with u = s / c; else rx = -i s (X + i u I)
+// with u = c / s.  Either form costs 4 FMAs per pair instead of 4 products and 4 FMAs; the factor has
+// modulus max(|c|, |s|) >= 1/sqrt 2 and, a global scale and phase of the whole layer, drops out of
+// <C> once the sum is multiplied by the product of the squared moduli (max(c^2, s^2) per qubit).
+struct RxScaled {
+    double u;
+    int swap;   // 1: the -i s (X + i u I) form
+    __device__ RxScaled(double c, double s) {
+        swap = fabs(c) < fabs(s);
+        u = swap ? c / s : s / c;
+    }
+};
+
+template <int SWAP>
+__device__ __forceinline__ void rx_pair(cd &a, cd &b, double u) {
+    if constexpr (!SWAP) {
+        const cd na = cmk(fma(u, b.y, a.x), fma(-u, b.x, a.y));
+        const cd nb = cmk(fma(u, a.y, b.x), fma(-u, a.x, b.y));
+        a = na;
+        b = nb;
+    } else {
+        const cd na = cmk(fma(-u, a.y, b.x), fma(u, a.x, b.y));
+        const cd nb = cmk(fma(-u, b.y, a.x), fma(u, b.x, a.y));
+        a = na;
+        b = nb;
+    }
 }
 
 // psi *= exp(-i C t) (circ_pure.c:27-32)
@@ -74,8 +96,8 @@ constexpr int padded(int n) { return n + (n >> 3); }
 // order, up to three qubits per pass: each thread holds the 8 amplitudes of one
 // 3-bit group in registers and applies the three rx in qubit order, so every
 // amplitude sees the same sequence of operations as gate-by-gate application.
-template <int THREADS>
-__device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, double c, double s) {
+template <int THREADS, int SWAP>
+__device__ __forceinline__ void lds_rx_form(cd *t, int lt, int b0, int b1, double u) {
     for (int b = b0; b < b1;) {
         const int r = (b1 - b) < 3 ? (b1 - b) : 3;
         const int R = 1 << r;
@@ -92,7 +114,7 @@ __device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, double c, 
                 if (k >= r) break;
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    if (j < R && !((j >> k) & 1)) rx_pair(v[j], v[j | (1 << k)], c, s);
+                    if (j < R && !((j >> k) & 1)) rx_pair<SWAP>(v[j], v[j | (1 << k)], u);
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j)
@@ -101,6 +123,13 @@ __device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, double c, 
         __syncthreads();
         b += r;
     }
+}
+
+// one form per call (the form is the same for every thread), so each instantiation holds one butterfly
+template <int THREADS>
+__device__ __forceinline__ void lds_rx(cd *t, int lt, int b0, int b1, const RxScaled &rx) {
+    if (rx.swap) lds_rx_form<THREADS, 1>(t, lt, b0, b1, rx.u);
+    else lds_rx_form<THREADS, 0>(t, lt, b0, b1, rx.u);
 }
 
 template <int THREADS>
@@ -140,16 +169,24 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
         const double *ang = A.angles + ev.angle_off;
         const double amp = 1.0 / sqrt((double)N);
         double acc = 0.0;
+        // product over layers of the squared modulus the scaled rx leave out, once per qubit
+        double scale = 1.0;
+        for (int l = 0; l < ev.p; ++l) {
+            const double c = ang[3 * l + 1], s = ang[3 * l + 2];
+            const double f2 = fabs(c) < fabs(s) ? s * s : c * c;
+            for (int q = 0; q < n; ++q) scale *= f2;
+        }
 
         if (n <= LT) {
             // ---- whole state in LDS ----
             for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[pad(x)] = cmk(amp, 0.0);
             __syncthreads();
             for (int l = 0; l < ev.p; ++l) {
-                const double g = ang[3 * l], c = ang[3 * l + 1], s = ang[3 * l + 2];
+                const double g = ang[3 * l];
+                const RxScaled rx(ang[3 * l + 1], ang[3 * l + 2]);
                 for (int x = threadIdx.x; x < (int)N; x += THREADS) tile[pad(x)] = phase(tile[pad(x)], C[x], g);
                 __syncthreads();
-                lds_rx<THREADS>(tile, n, 0, n, c, s);
+                lds_rx<THREADS>(tile, n, 0, n, rx);
             }
             for (int x = threadIdx.x; x < (int)N; x += THREADS) {
                 const cd v = tile[pad(x)];
@@ -170,9 +207,9 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
             for (int k = 0; k <= ev.p; ++k) {
                 const bool btile = k & 1;
                 const bool last = (k == ev.p);
-                const double g = last ? 0.0 : ang[3 * k], c = last ? 0.0 : ang[3 * k + 1],
-                             s = last ? 0.0 : ang[3 * k + 2];
-                const double cp = k ? ang[3 * (k - 1) + 1] : 0.0, sp = k ? ang[3 * (k - 1) + 2] : 0.0;
+                const double g = last ? 0.0 : ang[3 * k];
+                const RxScaled rx(last ? 1.0 : ang[3 * k + 1], last ? 0.0 : ang[3 * k + 2]);
+                const RxScaled rxp(k ? ang[3 * (k - 1) + 1] : 1.0, k ? ang[3 * (k - 1) + 2] : 0.0);
                 for (u64 ch = 0; ch < nchunk; ++ch) {
                     const u64 lo0 = ch << ls;
                     // global index of tile element x
@@ -183,7 +220,7 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
                     } else {
                         for (int x = threadIdx.x; x < T; x += THREADS) tile[pad(x)] = slot[QIDX(x)];
                         __syncthreads();
-                        lds_rx<THREADS>(tile, LT, ls, LT, cp, sp);
+                        lds_rx<THREADS>(tile, LT, ls, LT, rxp);
                         if (!last) {
                             for (int x = threadIdx.x; x < T; x += THREADS)
                                 tile[pad(x)] = phase(tile[pad(x)], C[QIDX(x)], g);
@@ -191,7 +228,7 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
                         }
                     }
                     if (!last) {
-                        lds_rx<THREADS>(tile, LT, 0, LT, c, s);
+                        lds_rx<THREADS>(tile, LT, 0, LT, rx);
                         for (int x = threadIdx.x; x < T; x += THREADS) slot[QIDX(x)] = tile[pad(x)];
                     } else {
                         for (int x = threadIdx.x; x < T; x += THREADS) {
@@ -205,7 +242,7 @@ __global__ __launch_bounds__(THREADS) void qaoa_persistent(QaoaArgs A) {
             }
         }
         const double tot = block_sum<THREADS>(acc, red);
-        if (threadIdx.x == 0) A.out[e] = tot;
+        if (threadIdx.x == 0) A.out[e] = tot * scale;
         __syncthreads();
     }
 }
