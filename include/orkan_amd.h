@@ -139,7 +139,9 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * states, bit 7 lds_bytes is the paired kernel's cap instead; 22: density-matrix permutations whose qubits
  * all lie at or above the tile (32 x 32 blocks for packed), which move whole tiles: flags bits 0 / 5 as
  * below, bit 1 sends them (and those whose qubits all lie below 5, which permute each 32 x 32 tile
- * within itself) to the tile-group kernel instead, bit 2 nontemporal accesses (tiled layout);
+ * within itself, and cx / cy / ccx, which otherwise run through the streamed kernels on their target)
+ * to the tile-group kernel instead, bit 2 nontemporal accesses (tiled layout), bits 3-4 log4 of the
+ * consecutive tiles each workgroup walks (0..2);
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad

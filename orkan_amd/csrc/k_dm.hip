@@ -54,6 +54,11 @@ struct DmOp {
     cd m[64];       // SOP: 4x4 row-major; DENSE: 2^k x 2^k row-major
     const cd *s2;   // SOP2: 16x16 row-major superoperator in device memory
     u64 still;      // MONO: bit a * 8 + b set when entries with gate bits (a, b) keep their value
+    // controlled single-target permutation run by the streamed 1-qubit kernels (cx, cy, ccx whose qubits
+    // straddle the tile): the quad's rows go through U when the row index has every bit of cmask, its
+    // columns when the column index has; U = [[0, i^pu01], [i^pu10, 0]] on the target
+    int ctl, pu01, pu10;
+    u64 cmask;
 };
 
 __device__ __forceinline__ u64 spread_bits(int a, const int *pos, int k) {
@@ -377,6 +382,32 @@ __device__ __forceinline__ void sop4(const DmOp &op, cd v0, cd v1, cd v2, cd v3,
 
 __device__ __forceinline__ u64 tile_base(u64 TR, u64 TC, int L) { return (TR * (TR + 1) / 2 + TC) << (2 * L); }
 
+// the quad (r0, r0 | 2^q) x (c0, c0 | 2^q) through the superoperator, or through the controlled
+// permutation A V B^dagger (A = U when r0 has every control bit, else I; B likewise for c0), which
+// moves entries exactly (multiplications by i^k only)
+__device__ __forceinline__ void apply4(const DmOp &op, u64 r0, u64 c0, cd v0, cd v1, cd v2, cd v3, cd &o0, cd &o1,
+                                       cd &o2, cd &o3) {
+    if (!op.ctl) {
+        sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+        return;
+    }
+    cd w0 = v0, w1 = v1, w2 = v2, w3 = v3;
+    if ((r0 & op.cmask) == op.cmask) {   // rows: (U V)[0] = u01 V[1], (U V)[1] = u10 V[0]
+        w0 = cipow(v2, op.pu01);
+        w1 = cipow(v3, op.pu01);
+        w2 = cipow(v0, op.pu10);
+        w3 = cipow(v1, op.pu10);
+    }
+    if ((c0 & op.cmask) == op.cmask) {   // columns: (M U^dagger)[., 0] = M[., 1] conj u01, [., 1] = M[., 0] conj u10
+        const cd a0 = w0, a2 = w2;
+        w0 = cipow(w1, -op.pu01);
+        w1 = cipow(a0, -op.pu10);
+        w2 = cipow(w3, -op.pu01);
+        w3 = cipow(a2, -op.pu10);
+    }
+    o0 = w0; o1 = w1; o2 = w2; o3 = w3;
+}
+
 template <typename S, int U, int NT>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     const int L = sp.L, q = sp.m, hl = L - 1;
@@ -405,7 +436,15 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
     for (int u = 0; u < U; ++u) {
         if (!ok[u]) continue;
         cd o0, o1, o2, o3;
-        sop4(op, v[u][0], v[u][1], v[u][2], v[u][3], o0, o1, o2, o3);
+        u64 r0 = 0, c0 = 0;
+        if (op.ctl) {   // full row and column of the quad's base entry
+            const u64 Q = (w * U + u) * DM_THREADS + threadIdx.x;
+            u64 TR, TC;
+            tri_decode(Q >> (2 * hl), TR, TC);
+            r0 = (TR << L) | (u64)(a00[u] >> L);
+            c0 = (TC << L) | (u64)(a00[u] & ((1 << L) - 1));
+        }
+        apply4(op, r0, c0, v[u][0], v[u][1], v[u][2], v[u][3], o0, o1, o2, o3);
         stx<NT>(t[u] + a00[u], o0);
         stx<NT>(t[u] + (a00[u] | mb), o1);
         stx<NT>(t[u] + (a00[u] | (mb << L)), o2);
@@ -454,7 +493,16 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__res
         // component-wise selects (a select of the vector type itself goes through memory)
         const cd v0 = sel(lo, a[u], a2), v1 = sel(lo, a2, a[u]), v2 = sel(lo, b[u], b2), v3 = sel(lo, b2, b[u]);
         cd o0, o1, o2, o3;
-        sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+        u64 r0 = 0, c0 = 0;
+        if (op.ctl) {   // full row and column of the quad's base entry (column bit q cleared)
+            const u64 Q = (w * U + u) * DM_THREADS + threadIdx.x;
+            const int within = (int)(Q & (((u64)1 << per_tile_log) - 1));
+            u64 TR, TC;
+            tri_decode(Q >> per_tile_log, TR, TC);
+            r0 = (TR << L) | ins0((u64)(within >> L), q);
+            c0 = (TC << L) | (u64)((within & (T - 1)) & ~mb);
+        }
+        apply4(op, r0, c0, v0, v1, v2, v3, o0, o1, o2, o3);
         stx<NT>(pa[u], sel(lo, o0, o1));
         stx<NT>(pb[u], sel(lo, o2, o3));
     }
@@ -492,7 +540,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             for (int e = 0; e < E; ++e) {
                 const int o = o0 + tid + e * DM_THREADS;
                 cd p0, p1, p2, p3;
-                sop4(op, va[e], vb[e], vc[e], vd[e], p0, p1, p2, p3);
+                apply4(op, (TR << L) | (u64)(o >> L), (TC << L) | (u64)(o & (T - 1)), va[e], vb[e], vc[e], vd[e], p0, p1,
+                       p2, p3);
                 stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
             }
             return;
@@ -504,10 +553,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             cd a1 = a0, b1 = b0, c1 = c0, d1 = d0;
             if (two) { a1 = ldx<NT>(A + o2); b1 = ldx<NT>(B + o2); c1 = ldx<NT>(C + o2); d1 = ldx<NT>(D + o2); }
             cd p0, p1, p2, p3;
-            sop4(op, a0, b0, c0, d0, p0, p1, p2, p3);
+            apply4(op, (TR << L) | (u64)(o >> L), (TC << L) | (u64)(o & (T - 1)), a0, b0, c0, d0, p0, p1, p2, p3);
             stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
             if (two) {
-                sop4(op, a1, b1, c1, d1, p0, p1, p2, p3);
+                apply4(op, (TR << L) | (u64)(o2 >> L), (TC << L) | (u64)(o2 & (T - 1)), a1, b1, c1, d1, p0, p1, p2, p3);
                 stx<NT>(A + o2, p0); stx<NT>(B + o2, p1); stx<NT>(C + o2, p2); stx<NT>(D + o2, p3);
             }
         }
@@ -539,7 +588,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             // (j, i) of the LDS copy is read and then rewritten by this thread alone
             const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
             cd o0, o1, o2, o3;
-            sop4(op, va[e], v1, vc[e], vd[e], o0, o1, o2, o3);
+            apply4(op, (TR << L) | (u64)i, (TC << L) | (u64)j, va[e], v1, vc[e], vd[e], o0, o1, o2, o3);
             stx<NT>(A + o, o0);
             stx<NT>(C + o, o2);
             stx<NT>(D + o, o3);
@@ -578,7 +627,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
             const cd v2 = diag ? vx[e] : vc[e];
             cd o0, o1, o2, o3;
-            sop4(op, va[e], v1, v2, vd[e], o0, o1, o2, o3);
+            apply4(op, (TR << L) | (u64)i, (TC << L) | (u64)j, va[e], v1, v2, vd[e], o0, o1, o2, o3);
             stx<NT>(A + o, o0);
             stx<NT>(C + o, o2);
             stx<NT>(D + o, o3);
@@ -602,7 +651,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
         const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
         const cd v2 = diag ? xs[i * LDS_STRIDE + j] : ldx<NT>(C + o);
         cd o0, o1, o2, o3;
-        sop4(op, v0, v1, v2, v3, o0, o1, o2, o3);
+        apply4(op, (TR << L) | (u64)i, (TC << L) | (u64)j, v0, v1, v2, v3, o0, o1, o2, o3);
         stx<NT>(A + o, o0);
         stx<NT>(C + o, o2);
         stx<NT>(D + o, o3);
@@ -741,7 +790,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             if (kind[h] == 2) v[h][1] = cconj(ms[x * LDS_STRIDE + y]);
             else if (kind[h] == 3) v[h][1] = cconj(v[h][2]);
             cd o0, o1, o2, o3;
-            sop4(op, v[h][0], v[h][1], v[h][2], v[h][3], o0, o1, o2, o3);
+            apply4(op, ins0(A * 32 + (u64)x, q), ins0(Bk * 32 + (u64)y, q), v[h][0], v[h][1], v[h][2], v[h][3], o0, o1,
+                   o2, o3);
             sth<NT>(p0[h], o0, own[h] & 1);
             sth<NT>(p2[h], o2, (own[h] >> 2) & 1);
             sth<NT>(p3[h], o3, (own[h] >> 3) & 1);
@@ -781,6 +831,8 @@ struct PermTilePlan {
     int pinv[8];  // gate bits a take the tile whose gate bits are pinv[a] ...
     int ph[8];    // ... times i^(ph[pinv a] - ph[pinv b])
     int xcd;
+    int tpw;      // consecutive tiles per workgroup
+    u64 nwg;      // workgroups of the launch
 };
 
 __device__ __forceinline__ u64 perm_src_tile(const PermTilePlan &p, u64 t, int &a_out) {
@@ -820,75 +872,82 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
     __shared__ cd ts[W_MAX * LDS_STRIDE];
     constexpr int E = W_MAX * W_MAX / DM_THREADS;   // entries of each tile per thread
     const int T = LAYOUT == ORK_TILED ? (1 << p.L) : 32, TT = T * T, tid = threadIdx.x;
-    const u64 w = remap_group(blockIdx.x, p.ntiles, p.xcd);
-    if (w >= p.ntiles) return;
-    u64 TR, TC;
-    tri_decode(w, TR, TC);
-    int sa, sb;
-    const u64 PR = perm_src_tile(p, TR, sa), PC = perm_src_tile(p, TC, sb);
-    const bool trans = PR < PC;
-    const u64 pr = trans ? PC : PR, pc = trans ? PR : PC;
-    const u64 wp = pr * (pr + 1) / 2 + pc;
-    if (wp < w) return;   // the workgroup of p swaps both
-    const int phs = (p.ph[sa] - p.ph[sb]) & 3;
-    if (wp == w && !trans && phs == 0) return;   // the tile keeps its value
-    int pa, pb;
-    (void)perm_src_tile(p, pr, pa);
-    (void)perm_src_tile(p, pc, pb);
-    const int php = (p.ph[pa] - p.ph[pb]) & 3;
-    const bool pair = wp != w;
-    cd vs[E], vp[E];
-    u64 os[E], op[E];
-    bool ok[E];
-    int ii[E], jj[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int o = tid + e * DM_THREADS;
-        bool st = false;
-        int i = 0, j = 0;
-        u64 off = 0, offp = 0;
-        if (o < TT) perm_tile_entry<LAYOUT>(p, TR, TC, o, i, j, off, st);
-        ok[e] = st;
-        ii[e] = i;
-        jj[e] = j;
-        os[e] = off;
-        if (st && pair) {
-            bool stp;
-            int i2, j2;
-            perm_tile_entry<LAYOUT>(p, pr, pc, o, i2, j2, offp, stp);
-        }
-        op[e] = offp;
-        vs[e] = st ? ldx<NTA>(rho + off) : cmk(0.0, 0.0);
-        vp[e] = (st && pair) ? ldx<NTA>(rho + offp) : cmk(0.0, 0.0);
-    }
-    if (!trans) {
-        // same in-tile positions: s takes p's entries and p takes s's (or s its own, phase only)
+    // p.tpw consecutive tiles per workgroup: most tiles are left at once (a non-owner of its pair, or a
+    // tile that keeps its value), which costs a loop iteration here instead of a workgroup dispatch
+    const u64 g = remap_group(blockIdx.x, p.nwg, p.xcd);
+    for (int tw = 0; tw < p.tpw; ++tw) {
+        const u64 w = g * (u64)p.tpw + (u64)tw;
+        if (w >= p.ntiles) break;
+        u64 TR, TC;
+        tri_decode(w, TR, TC);
+        int sa, sb;
+        const u64 PR = perm_src_tile(p, TR, sa), PC = perm_src_tile(p, TC, sb);
+        const bool trans = PR < PC;
+        const u64 pr = trans ? PC : PR, pc = trans ? PR : PC;
+        const u64 wp = pr * (pr + 1) / 2 + pc;
+        if (wp < w) continue;   // the workgroup of p swaps both
+        const int phs = (p.ph[sa] - p.ph[sb]) & 3;
+        if (wp == w && !trans && phs == 0) continue;   // the tile keeps its value
+        int pa, pb;
+        (void)perm_src_tile(p, pr, pa);
+        (void)perm_src_tile(p, pc, pb);
+        const int php = (p.ph[pa] - p.ph[pb]) & 3;
+        const bool pair = wp != w;
+        cd vs[E], vp[E];
+        u64 os[E], op[E];
+        bool ok[E];
+        int ii[E], jj[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            if (!ok[e]) continue;
-            stx<NTA>(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
-            if (pair) stx<NTA>(rho + op[e], cipow(vs[e], php));
+            const int o = tid + e * DM_THREADS;
+            bool st = false;
+            int i = 0, j = 0;
+            u64 off = 0, offp = 0;
+            if (o < TT) perm_tile_entry<LAYOUT>(p, TR, TC, o, i, j, off, st);
+            ok[e] = st;
+            ii[e] = i;
+            jj[e] = j;
+            os[e] = off;
+            if (st && pair) {
+                bool stp;
+                int i2, j2;
+                perm_tile_entry<LAYOUT>(p, pr, pc, o, i2, j2, offp, stp);
+            }
+            op[e] = offp;
+            vs[e] = st ? ldx<NTA>(rho + off) : cmk(0.0, 0.0);
+            vp[e] = (st && pair) ? ldx<NTA>(rho + offp) : cmk(0.0, 0.0);
         }
-        return;
+        if (!trans) {
+            // same in-tile positions: s takes p's entries and p takes s's (or s its own, phase only)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (!ok[e]) continue;
+                stx<NTA>(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
+                if (pair) stx<NTA>(rho + op[e], cipow(vs[e], php));
+            }
+            continue;
+        }
+        // new s (i, j) = i^phs conj(old p (j, i)), new p (i, j) = i^php conj(old s (j, i)); off the
+        // diagonal every entry of both tiles is stored, and every old value is already in registers
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (ok[e]) stx<NTA>(rho + os[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), phs));
+        __syncthreads();
+        if (pair) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = vs[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (ok[e]) stx<NTA>(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
+            __syncthreads();   // before the next tile's LDS writes
+        }
     }
-    // new s (i, j) = i^phs conj(old p (j, i)), new p (i, j) = i^php conj(old s (j, i)); off the diagonal
-    // every entry of both tiles is stored, and every old value is already in registers
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (ok[e]) stx<NTA>(rho + os[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), phs));
-    if (!pair) return;
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = vs[e];
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (ok[e]) stx<NTA>(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
 }
 
 // ---- permutations inside 32 x 32 tiles: every gate qubit below 5 -------------------------------
@@ -1358,6 +1417,33 @@ bool perm_tiles_ok(int layout, int L, int n, const ork_mono_t *m) {
     return true;
 }
 
+// Is the gate U on one target qubit controlled by the others (cx, cy, ccx)?  Then tq = the target,
+// U = [[0, i^pu01], [i^pu10, 0]] and cmask the control qubits as bits of the row / column index.
+bool controlled_single_target(const ork_mono_t *m, int &tq, int &pu01, int &pu10, u64 &cmask) {
+    const int k = m->k, K = 1 << k;
+    int flip = 0;
+    for (int a = 0; a < K; ++a) flip |= a ^ m->perm[a];
+    if (flip == 0 || (flip & (flip - 1))) return false;   // exactly one target bit moves
+    int tj = 0;
+    while (!((flip >> tj) & 1)) ++tj;
+    const int cl = (K - 1) & ~flip;   // control bits (local)
+    pu01 = pu10 = 0;
+    for (int a = 0; a < K; ++a) {
+        if ((a & cl) == cl) {
+            if (m->perm[a] != (a ^ flip)) return false;
+            if ((a >> tj) & 1) pu01 = m->ph[a] & 3;   // U |1> = i^pu01 |0>
+            else pu10 = m->ph[a] & 3;                 // U |0> = i^pu10 |1>
+        } else if (m->perm[a] != a || (m->ph[a] & 3)) {
+            return false;
+        }
+    }
+    tq = m->qs[tj];
+    cmask = 0;
+    for (int j = 0; j < k; ++j)
+        if (j != tj) cmask |= (u64)1 << m->qs[j];
+    return true;
+}
+
 // can the gate run through dm_perm_intile_kernel?  Every qubit below 5, 32 x 32 tiles (tiled with
 // LOG_TILE_DIM 5, or packed blocks) and at least one whole tile
 bool perm_intile_ok(int layout, int L, int n, const ork_mono_t *m) {
@@ -1416,6 +1502,8 @@ int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t 
     }
     const Shape sh = g_permt_shape;
     p.xcd = sh.xcd;
+    p.tpw = 1 << (2 * sh.lp);   // flags bits 3-4: log4 of the tiles per workgroup
+    p.nwg = (p.ntiles + p.tpw - 1) / p.tpw;
     S *r = (S *)rho;
     // nontemporal accesses (flags bit 2) on the tiled layout, whose tiles are whole lines, on states of at
     // least the nontemporal minimum; the packed layout's block columns share lines with their neighbours
@@ -1423,8 +1511,8 @@ int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t 
                      ((u64)sizeof(S) << p.L) >= 128;
     const int stat = (int)sizeof(cd) * W_MAX * LDS_STRIDE;
     const int lds = sh.lds > stat ? sh.lds - stat : 0;
-    if (ntm) dm_perm_tiles_kernel<S, LAYOUT, 1><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>(r, p);
-    else dm_perm_tiles_kernel<S, LAYOUT, 0><<<(unsigned)p.ntiles, DM_THREADS, lds, st>>>(r, p);
+    if (ntm) dm_perm_tiles_kernel<S, LAYOUT, 1><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
+    else dm_perm_tiles_kernel<S, LAYOUT, 0><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
     ORK_LAUNCH_CHECK();
     return 0;
 }
@@ -1600,6 +1688,8 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
             if (op.srt[b] < op.srt[a]) { int t = op.srt[a]; op.srt[a] = op.srt[b]; op.srt[b] = t; }
     for (int a = 0; a < 8; ++a) { op.pinv[a] = a; op.ph[a] = 0; }
     op.still = 0;
+    op.ctl = op.pu01 = op.pu10 = 0;
+    op.cmask = 0;
     if (opk == ORK_DM_MONO) {
         for (int a = 0; a < (1 << k); ++a) {
             if (m->perm[a] < 0 || m->perm[a] >= (1 << k)) return (int)hipErrorInvalidValue;
@@ -1618,6 +1708,26 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     op.s2 = opk == ORK_DM_SOP2 ? (const cd *)mat : nullptr;
     if (opk == ORK_DM_SOP2 && k != 2) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
+    // a controlled gate with one target (cx, cy, ccx) runs through the streamed 1-qubit kernels on that
+    // target, which move every entry once: measured on 14 qubits this beats the whole-tile and in-tile
+    // permutations for cx / cy everywhere (3/4 of the entries move: 4.4-4.9 against 3.5-4.8 TB/s tiled,
+    // 3.6-4.2 against 3.0-3.8 packed) and for ccx unless every qubit is at or above the tile, where only
+    // its 7/16 moved entries are worth moving (whole tiles 3.5-4.4 against 2.6-2.7 TB/s)
+    int tq, pu01, pu10;
+    u64 cm;
+    if (opk == ORK_DM_MONO && g_stream_on && controlled_single_target(m, tq, pu01, pu10, cm) &&
+        !(m->k == 3 && perm_tiles_ok(layout, log_tile, n, m)) && !g_permt_shape.col &&
+        ((layout == ORK_PACKED && n >= 6) || (layout == ORK_TILED && n >= log_tile))) {
+        DmOp c = op;
+        c.ctl = 1;
+        c.cmask = cm;
+        c.pu01 = pu01;
+        c.pu10 = pu10;
+        if (layout == ORK_PACKED)
+            return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, tq, c) : launch_pstream<double2>(st, rho, n, tq, c);
+        return prec == ORK_F32 ? launch_stream<float2>(st, rho, log_tile, n, tq, c)
+                               : launch_stream<double2>(st, rho, log_tile, n, tq, c);
+    }
     if (opk == ORK_DM_MONO && perm_intile_ok(layout, log_tile, n, m)) {
         if (prec == ORK_F32)
             return layout == ORK_PACKED ? launch_perm_intile<float2, ORK_PACKED>(st, rho, n, m)

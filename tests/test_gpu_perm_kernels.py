@@ -2,10 +2,12 @@
 results.  Statevector: the lane-exchange kernel (every gate qubit below 6), the paired complex64
 kernel (no gate qubit at bit 0, two amplitudes per 16 B access) and the per-group kernel, under every
 launch shape of tuning hook 21, with nontemporal accesses forced on through hook 13.  Density matrix:
-x and y run through the streamed superoperator kernels (their superoperators hold 0, +-1, +-i only)
-and cx / cy / swap / ccx through the tile-group kernel, which leaves unchanged entries unread and
-unwritten; both are checked against the oracle on every position, and the streamed x / y against
-the tile-group kernel bit for bit (hook 8 switches the streamed kernels off)."""
+x and y run through the streamed superoperator kernels (their superoperators hold 0, +-1, +-i only),
+cx / cy / ccx through the same kernels on their target as a controlled permutation of each quad
+(ccx with every qubit at or above the tile excepted), swap and that ccx through the whole-tile or
+in-tile permutation kernels where their qubits allow, everything else through the tile-group kernel,
+which leaves unchanged entries unread and unwritten.  Each path is checked against the oracle and
+bit for bit against the tile-group kernel (hooks 8 and 22 switch the others off)."""
 import ctypes as C
 
 import numpy as np
@@ -87,9 +89,10 @@ def test_sv_permutation_kernels_bit_identical(eng, fp32):
 @pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
 @pytest.mark.parametrize("log_tile", [5, 3])
 def test_dm_permutations_against_oracle(kind, log_tile):
-    """Every permutation gate on many positions against the oracle, with the tile-group kernel's loads
-    one after the other (the default) and all in flight at once (group shape flag bit 6): the two
-    forms are bit-identical."""
+    """Every permutation gate on many positions against the oracle through the default dispatch, and
+    the same sequence with every gate sent to the tile-group kernel (hooks 8 and 22), its loads one
+    after the other (the default) and all in flight at once (group shape flag bit 6): all three
+    bit-identical."""
     n = 8
     dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
     api = dev.api
@@ -105,12 +108,15 @@ def test_dm_permutations_against_oracle(kind, log_tile):
     want = orc.read(so)
     orc.free(so)
     outs = []
-    for hoist in (0, 64):
+    for hoist in (None, 0, 64):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
-        if kind == MIXED_PACKED:
-            assert api.orkan_set_launch_shape(GROUP_PACKED_HOOK, 1 | 2 | hoist, 0) == 0
-        else:
-            assert api.orkan_set_launch_shape(GROUP_TILED_HOOK, 1 | hoist, 0) == 0
+        if hoist is not None:
+            assert api.orkan_set_launch_shape(STREAM_HOOK, 0, 0) == 0
+            assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | 2, 0) == 0
+            if kind == MIXED_PACKED:
+                assert api.orkan_set_launch_shape(GROUP_PACKED_HOOK, 1 | 2 | hoist, 0) == 0
+            else:
+                assert api.orkan_set_launch_shape(GROUP_TILED_HOOK, 1 | hoist, 0) == 0
         sd = dev.new(kind, n, rho)
         for name, qs in ops:
             dev.gate(sd, name, *qs)
@@ -118,7 +124,7 @@ def test_dm_permutations_against_oracle(kind, log_tile):
         dev.free(sd)
     assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
     assert float(np.abs(outs[0] - want).max()) <= 1e-12
-    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
 
 
 @pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
@@ -223,6 +229,43 @@ def test_dm_intile_permutations_equal_tile_groups(kind, n):
     for off in (0, 2):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
         assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | 4 | off, 0) == 0
+        st = dev.new(kind, n, rho)
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        outs.append(dev.read(st))
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert np.array_equal(outs[0], outs[1])
+    so = orc.new(kind, n, rho)
+    for name, qs in ops:
+        orc.gate(so, name, *qs)
+    assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
+    orc.free(so)
+
+
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+@pytest.mark.parametrize("log_tile", [5, 3])
+def test_dm_straddling_controlled_gates(kind, log_tile):
+    """cx, cy and ccx whose qubits straddle the tile (32 x 32 blocks for packed) run through the streamed
+    1-qubit kernels on their target as a controlled permutation of each quad; hook 8 (lds_bytes 0) sends
+    them to the tile-group kernel instead.  Both move entries exactly: bit-identical, and equal to the
+    oracle, with targets inside the tile (lane-exchange and quad kernels) and above it (tile quads)."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
+    api = dev.api
+    orc = engines.oracle(log_tile)
+    T = log_tile if kind == MIXED_TILED else 5
+    n = T + 3
+    ops = []
+    for lo in range(T):
+        for hi in range(T, n):
+            ops += [("cx", (lo, hi)), ("cx", (hi, lo)), ("cy", (lo, hi)), ("cy", (hi, lo))]
+    ops += [("ccx", (0, T, 1)), ("ccx", (T - 1, T + 1, T)), ("ccx", (T, 1, T + 2)), ("ccx", (0, T - 1, T + 1)),
+            ("ccx", (T + 2, T, 0)), ("ccx", (1, 2, T + 1)), ("swap_gate", (T - 1, T))]
+    rho = to_layout(random_rho(n, np.random.default_rng(300 + kind + log_tile)), kind, log_tile)
+    outs = []
+    for streamed in (1, 0):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(STREAM_HOOK, 0, streamed) == 0
         st = dev.new(kind, n, rho)
         for name, qs in ops:
             dev.gate(st, name, *qs)
