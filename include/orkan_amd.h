@@ -141,7 +141,12 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * below, bit 1 sends them (and those whose qubits all lie below 5, which permute each 32 x 32 tile
  * within itself, and cx / cy / ccx, which otherwise run through the streamed kernels on their target)
  * to the tile-group kernel instead, bit 2 nontemporal accesses (tiled layout), bits 3-4 log4 of the
- * consecutive tiles each workgroup walks (0..2);
+ * consecutive tiles each workgroup walks (0..2), bit 6 moves packed cx / cy / ccx with every qubit at
+ * or above the 32 x 32 blocks as whole blocks (else they stream where their unchanged entries can be
+ * left alone); 23 / 24 / 25: cx and cy on packed states / controlled
+ * gates on tiled states / ccx on packed states leave the entries they do not change unread and
+ * unwritten when their lowest control qubit is at least the bits of a 128 B line (3 complex128, 4
+ * complex64) plus flags (0..63, 63: never; defaults 2 / 0 / 0; lds_bytes must be 0);
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad

@@ -54,10 +54,12 @@ struct DmOp {
     cd m[64];       // SOP: 4x4 row-major; DENSE: 2^k x 2^k row-major
     const cd *s2;   // SOP2: 16x16 row-major superoperator in device memory
     u64 still;      // MONO: bit a * 8 + b set when entries with gate bits (a, b) keep their value
-    // controlled single-target permutation run by the streamed 1-qubit kernels (cx, cy, ccx whose qubits
-    // straddle the tile): the quad's rows go through U when the row index has every bit of cmask, its
-    // columns when the column index has; U = [[0, i^pu01], [i^pu10, 0]] on the target
-    int ctl, pu01, pu10;
+    // controlled single-target permutation run by the streamed 1-qubit kernels (cx, cy, ccx): the quad's
+    // rows go through U when the row index has every bit of cmask, its columns when the column index
+    // has; U = [[0, i^pu01], [i^pu10, 0]] on the target.  skip: quads with neither (which keep their
+    // value) are neither read nor written -- set where every control bit lies above the bits of a
+    // 128 B line, so that the quads left out cover whole lines
+    int ctl, pu01, pu10, skip;
     u64 cmask;
 };
 
@@ -408,7 +410,16 @@ __device__ __forceinline__ void apply4(const DmOp &op, u64 r0, u64 c0, cd v0, cd
     o0 = w0; o1 = w1; o2 = w2; o3 = w3;
 }
 
-template <typename S, int U, int NT>
+// the quad keeps its value under the controlled permutation, and the operation leaves such quads alone;
+// the streamed kernels take SKIP as a template argument, so that their superoperator forms carry no
+// trace of the test (SKIP is set exactly when op.skip is)
+template <int SKIP>
+__device__ __forceinline__ bool quad_still(const DmOp &op, u64 r0, u64 c0) {
+    if constexpr (!SKIP) return false;
+    else return (r0 & op.cmask) != op.cmask && (c0 & op.cmask) != op.cmask;
+}
+
+template <typename S, int U, int NT, int SKIP>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     const int L = sp.L, q = sp.m, hl = L - 1;
     const int mb = 1 << q;
@@ -425,6 +436,11 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
         const int r0 = (int)ins0((u64)(i >> hl), q), c0 = (int)ins0((u64)(i & ((1 << hl) - 1)), q);
         t[u] = ok[u] ? rho + ((Q >> (2 * hl)) << (2 * L)) : rho;
         a00[u] = (r0 << L) | c0;
+        if (SKIP && ok[u]) {
+            u64 TR, TC;
+            tri_decode(Q >> (2 * hl), TR, TC);
+            ok[u] = !quad_still<SKIP>(op, (TR << L) | (u64)r0, (TC << L) | (u64)c0);
+        }
         if (ok[u]) {
             v[u][0] = ldx<NT>(t[u] + a00[u]);
             v[u][1] = ldx<NT>(t[u] + (a00[u] | mb));
@@ -460,7 +476,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
 // lanes walking the columns of a row so that every load and store instruction covers whole lines;
 // the partner column's two entries come from lane ^ 2^q.  Both lanes of a pair form the same four
 // products as the quad kernel above and each stores its own column.
-template <typename S, int U, int NT>
+template <typename S, int U, int NT, int SKIP>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     const int L = sp.L, q = sp.m, T = 1 << L, mb = 1 << q;
     const u64 w = remap_group(blockIdx.x, sp.nblocks, sp.xcd);
@@ -479,6 +495,11 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__res
         pb[u] = tile + (((r0 | mb) << L) | c);
         a[u] = cmk(0.0, 0.0);
         b[u] = a[u];
+        if (SKIP && ok[u]) {   // both lanes of a pair decide alike: their columns differ in bit q only
+            u64 TR, TC;
+            tri_decode(Q >> per_tile_log, TR, TC);
+            ok[u] = !quad_still<SKIP>(op, (TR << L) | (u64)r0, (TC << L) | (u64)(c & ~mb));
+        }
         if (ok[u]) {
             a[u] = ldx<NT>(pa[u]);
             b[u] = ldx<NT>(pb[u]);
@@ -508,7 +529,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__res
     }
 }
 
-template <typename S, int NT, int PARTS>
+template <typename S, int NT, int PARTS, int SKIP>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict__ rho, StreamPlan sp, DmOp op) {
     __shared__ cd xs[W_MAX * LDS_STRIDE];
     const int L = sp.L, T = 1 << L, TT = T * T, tid = threadIdx.x;
@@ -520,6 +541,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     tri_decode(w, Rr, Cr);
     const u64 bb = (u64)1 << sp.m;
     const u64 TR = ins0(Rr, sp.m), TC = ins0(Cr, sp.m);
+    // does the quad at in-tile (i, j) of the tile quad change?  (always, but for skipped controlled quads)
+    auto live = [&](int i, int j) { return !quad_still<SKIP>(op, (TR << L) | (u64)i, (TC << L) | (u64)j); };
     S *A = rho + tile_base(TR, TC, L);
     S *C = rho + tile_base(TR | bb, TC, L);
     S *D = rho + tile_base(TR | bb, TC | bb, L);
@@ -531,13 +554,16 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
             constexpr int E = 4 / PARTS;
             const int o0 = part * (4 * DM_THREADS / PARTS);
             cd va[E], vb[E], vc[E], vd[E];
+            bool lv[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int o = o0 + tid + e * DM_THREADS;
-                va[e] = ldx<NT>(A + o); vb[e] = ldx<NT>(B + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o);
+                lv[e] = live(o >> L, o & (T - 1));
+                if (lv[e]) { va[e] = ldx<NT>(A + o); vb[e] = ldx<NT>(B + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o); }
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
+                if (!lv[e]) continue;
                 const int o = o0 + tid + e * DM_THREADS;
                 cd p0, p1, p2, p3;
                 apply4(op, (TR << L) | (u64)(o >> L), (TC << L) | (u64)(o & (T - 1)), va[e], vb[e], vc[e], vd[e], p0, p1,
@@ -548,20 +574,24 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
         }
         for (int o = tid; o < TT; o += 2 * DM_THREADS) {
             const int o2 = o + DM_THREADS;
-            const bool two = o2 < TT;
-            const cd a0 = ldx<NT>(A + o), b0 = ldx<NT>(B + o), c0 = ldx<NT>(C + o), d0 = ldx<NT>(D + o);
-            cd a1 = a0, b1 = b0, c1 = c0, d1 = d0;
-            if (two) { a1 = ldx<NT>(A + o2); b1 = ldx<NT>(B + o2); c1 = ldx<NT>(C + o2); d1 = ldx<NT>(D + o2); }
+            const bool k0 = live(o >> L, o & (T - 1)), k1 = o2 < TT && live(o2 >> L, o2 & (T - 1));
+            cd a0 = cmk(0.0, 0.0), b0 = a0, c0 = a0, d0 = a0, a1 = a0, b1 = a0, c1 = a0, d1 = a0;
+            if (k0) { a0 = ldx<NT>(A + o); b0 = ldx<NT>(B + o); c0 = ldx<NT>(C + o); d0 = ldx<NT>(D + o); }
+            if (k1) { a1 = ldx<NT>(A + o2); b1 = ldx<NT>(B + o2); c1 = ldx<NT>(C + o2); d1 = ldx<NT>(D + o2); }
             cd p0, p1, p2, p3;
-            apply4(op, (TR << L) | (u64)(o >> L), (TC << L) | (u64)(o & (T - 1)), a0, b0, c0, d0, p0, p1, p2, p3);
-            stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
-            if (two) {
+            if (k0) {
+                apply4(op, (TR << L) | (u64)(o >> L), (TC << L) | (u64)(o & (T - 1)), a0, b0, c0, d0, p0, p1, p2, p3);
+                stx<NT>(A + o, p0); stx<NT>(B + o, p1); stx<NT>(C + o, p2); stx<NT>(D + o, p3);
+            }
+            if (k1) {
                 apply4(op, (TR << L) | (u64)(o2 >> L), (TC << L) | (u64)(o2 & (T - 1)), a1, b1, c1, d1, p0, p1, p2, p3);
                 stx<NT>(A + o2, p0); stx<NT>(B + o2, p1); stx<NT>(C + o2, p2); stx<NT>(D + o2, p3);
             }
         }
         return;
     }
+    // X: the stored mirror of B = (TR, TC | bb); its entry (j, i) is the conjugate of B's (i, j), which
+    // belongs to the quad at (i, j)
     const bool diag = (TR == TC);
     S *X = diag ? C : rho + tile_base(TC | bb, TR, L);
     if (PARTS > 1 && TT == 4 * DM_THREADS && !diag) {
@@ -569,21 +599,25 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
         constexpr int E = 4 / PARTS, R = 32 / PARTS;
         const int i0 = part * R;
         cd va[E], vc[E], vd[E], vx[E];
+        bool lq[E], lx[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int o = i0 * 32 + tid + e * DM_THREADS;
             const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
-            va[e] = ldx<NT>(A + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o);
-            vx[e] = ldx<NT>(X + xj * 32 + xi);
+            lq[e] = live(o >> 5, o & 31);
+            lx[e] = live(xi, xj);
+            if (lq[e]) { va[e] = ldx<NT>(A + o); vc[e] = ldx<NT>(C + o); vd[e] = ldx<NT>(D + o); }
+            if (lx[e]) vx[e] = ldx<NT>(X + xj * 32 + xi);
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
-            xs[xj * LDS_STRIDE + xi] = vx[e];
+            if (lx[e]) xs[xj * LDS_STRIDE + xi] = vx[e];
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
+            if (!lq[e]) continue;
             const int o = i0 * 32 + tid + e * DM_THREADS, i = o >> 5, j = o & 31;
             // (j, i) of the LDS copy is read and then rewritten by this thread alone
             const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
@@ -598,7 +632,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int idx = tid + e * DM_THREADS, xj = idx / R, xi = i0 + idx % R;
-            stx<NT>(X + xj * 32 + xi, xs[xj * LDS_STRIDE + xi]);
+            if (lx[e]) stx<NT>(X + xj * 32 + xi, xs[xj * LDS_STRIDE + xi]);
         }
         return;
     }
@@ -606,22 +640,29 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     if (TT == 4 * DM_THREADS) {
         // 32 x 32 tiles: every load of the workgroup (A, D, X and, off the diagonal, C) in flight at
         // once.  On the diagonal X is C itself, so a thread's own C entry comes from its registers
-        // and only the transposed one from LDS.
+        // and only the transposed one from LDS (whether a quad changes is then symmetric in (i, j)).
         cd va[4], vc[4], vd[4], vx[4];
+        bool lq[4], lx[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int o = tid + e * DM_THREADS;
-            va[e] = ldx<NT>(A + o); vd[e] = ldx<NT>(D + o); vx[e] = ldx<NT>(X + o);
-            if (!diag) vc[e] = ldx<NT>(C + o);
+            const int o = tid + e * DM_THREADS, i = o >> L, j = o & (T - 1);
+            lq[e] = live(i, j);
+            lx[e] = live(j, i);   // X's (i, j) belongs to the quad at (j, i)
+            if (lq[e]) {
+                va[e] = ldx<NT>(A + o); vd[e] = ldx<NT>(D + o);
+                if (!diag) vc[e] = ldx<NT>(C + o);
+            }
+            if (lx[e]) vx[e] = ldx<NT>(X + o);
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int o = tid + e * DM_THREADS;
-            xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = vx[e];
+            if (lx[e]) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = vx[e];
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+            if (!lq[e]) continue;
             const int o = tid + e * DM_THREADS, i = o >> L, j = o & (T - 1);
             // (j, i) of the LDS copy is read and then rewritten by this thread alone
             const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
@@ -638,15 +679,17 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int o = tid + e * DM_THREADS;
-                stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+                if (lx[e]) stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
             }
         }
         return;
     }
-    for (int o = tid; o < TT; o += DM_THREADS) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldx<NT>(X + o);
+    for (int o = tid; o < TT; o += DM_THREADS)
+        if (live(o & (T - 1), o >> L)) xs[(o >> L) * LDS_STRIDE + (o & (T - 1))] = ldx<NT>(X + o);
     __syncthreads();
     for (int o = tid; o < TT; o += DM_THREADS) {
         const int i = o >> L, j = o & (T - 1);
+        if (!live(i, j)) continue;
         const cd v0 = ldx<NT>(A + o), v3 = ldx<NT>(D + o);
         const cd v1 = cconj(xs[j * LDS_STRIDE + i]);
         const cd v2 = diag ? xs[i * LDS_STRIDE + j] : ldx<NT>(C + o);
@@ -659,7 +702,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_tiles_kernel(S *__restrict_
     }
     if (!diag) {
         __syncthreads();
-        for (int o = tid; o < TT; o += DM_THREADS) stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
+        for (int o = tid; o < TT; o += DM_THREADS)
+            if (live(o & (T - 1), o >> L)) stx<NT>(X + o, xs[(o >> L) * LDS_STRIDE + (o & (T - 1))]);
     }
 }
 
@@ -719,7 +763,7 @@ template <int NT, typename S> __device__ __forceinline__ void sth(S *p, cd v, bo
 
 // PARTS consecutive workgroups (after the XCD remap) share one 32 x 32 block, each taking 32 / PARTS
 // of its columns b (their quads and their mirror entries): no entry is touched by two of them.
-template <typename S, int NT, int PARTS>
+template <typename S, int NT, int PARTS, int SKIP>
 __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict__ rho, PackedStreamPlan pp, DmOp op) {
     __shared__ cd ms[32 * LDS_STRIDE];
     const int q = pp.q, tid = threadIdx.x;
@@ -731,6 +775,10 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
     const bool has_mirror = q >= 5 ? ((A >> (q - 5)) == (Bk >> (q - 5))) : (A == Bk);
     constexpr int QPT = 1024 / DM_THREADS / PARTS, YW = 32 / PARTS;
     const int y0 = part * YW;
+    // quad (a, b) holds a mirror entry (c1, r0), and changes it
+    auto mirror = [&](u64 a, u64 b) {
+        return a > b && (a >> q) == (b >> q) && !quad_still<SKIP>(op, ins0(a, q), ins0(b, q));
+    };
     // the block's mirror entries, loaded together with the quads below and staged through LDS
     // once every load is in flight
     cd mv[QPT];
@@ -739,8 +787,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
         for (int h = 0; h < QPT; ++h) {
             const int t = tid + DM_THREADS * h, x = t / YW, y = y0 + t % YW;   // lanes walk y: rows c1 of column r0(x)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
-            if (a > b && (a >> q) == (b >> q))
-                mv[h] = ldx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
+            if (mirror(a, b)) mv[h] = ldx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
     }
     // all quads of a thread in flight: t = tid + 256 h
@@ -754,6 +801,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             const int tt = tid + DM_THREADS * h, x = tt & 31, y = y0 + (tt >> 5);   // lanes walk x: rows r0 of column c0(y)
             const u64 a = A * 32 + x, b = Bk * 32 + y;
             kind[h] = a < b ? 0 : a == b ? 3 : ((a >> q) == (b >> q) ? 2 : 1);
+            if (kind[h] && quad_still<SKIP>(op, ins0(a, q), ins0(b, q))) kind[h] = 0;
             p0[h] = p1[h] = p2[h] = p3[h] = rho;
             own[h] = 0;
             if (kind[h]) {
@@ -779,7 +827,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             for (int h = 0; h < QPT; ++h) {
                 const int t = tid + DM_THREADS * h, x = t / YW, y = y0 + t % YW;
                 const u64 a = A * 32 + x, b = Bk * 32 + y;
-                if (a > b && (a >> q) == (b >> q)) ms[x * LDS_STRIDE + y] = mv[h];
+                if (mirror(a, b)) ms[x * LDS_STRIDE + y] = mv[h];
             }
             __syncthreads();
         }
@@ -804,7 +852,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
         for (int t = tid; t < 1024 / PARTS; t += DM_THREADS) {
             const int x = t / YW, y = y0 + t % YW;
             const u64 a = A * 32 + x, b = Bk * 32 + y;
-            if (a > b && (a >> q) == (b >> q))
+            if (mirror(a, b))
                 stx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)), ms[x * LDS_STRIDE + y]);
         }
     }
@@ -922,7 +970,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 if (!ok[e]) continue;
-                stx<NTA>(rho + os[e], cipow(pair ? vp[e] : vs[e], phs));
+                stx<NTA>(rho + os[e], cipow(sel(pair, vp[e], vs[e]), phs));
                 if (pair) stx<NTA>(rho + op[e], cipow(vs[e], php));
             }
             continue;
@@ -931,7 +979,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
         // diagonal every entry of both tiles is stored, and every old value is already in registers
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = pair ? vp[e] : vs[e];
+            if (ok[e]) ts[ii[e] * LDS_STRIDE + jj[e]] = sel(pair, vp[e], vs[e]);
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < E; ++e)
@@ -1295,9 +1343,16 @@ constexpr Shape kStreamShape[4] = {{2, 0, 81920, 1, 1}, {0, 0, 65536, 1, 2}, {0,
 constexpr Shape kLaneShape = {0, 0, 49152, 1, 0};
 Shape g_stream_shape[4] = {kStreamShape[0], kStreamShape[1], kStreamShape[2], kStreamShape[3]};
 int g_stream_on = 1;
+// controlled gates in the streamed kernels leave their unchanged quads alone when the lowest control
+// qubit is at least the line bits (3 for complex128, 4 for complex64) plus a margin: [packed cx / cy,
+// tiled, packed ccx].  Packed column runs start anywhere in a line, so each 2^c-row run of skipped
+// quads leaves partial lines at its ends: on 14 qubits that costs cx (1/4 of the quads skipped) 2-8 %
+// at controls 3-4, while ccx (9/16 skipped) gains 25-55 % there; tiled rows are line-aligned.
+constexpr int kSkipMargin[3] = {2, 0, 0};
+int g_skip_margin[3] = {kSkipMargin[0], kSkipMargin[1], kSkipMargin[2]};
 
-template <typename S>
-int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op) {
+template <typename S, int SKIP>
+int launch_stream_t(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op) {
     const u64 nt = (u64)1 << (n - L);
     StreamPlan sp;
     sp.L = L;
@@ -1314,7 +1369,7 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
         sp.nitems = (nt * (nt + 1) / 2) << (2 * L - 1);
         sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
-        dm_sop_intile_lane_kernel<S, U, 1><<<(unsigned)sp.nblocks, DM_THREADS, kLaneShape.lds / (U / 2), st>>>(
+        dm_sop_intile_lane_kernel<S, U, 1, SKIP><<<(unsigned)sp.nblocks, DM_THREADS, kLaneShape.lds / (U / 2), st>>>(
             (S *)rho, sp, op);
     } else if (q < L) {
         // complex64 storage: two quads per thread and half the LDS cap keep the bytes in flight
@@ -1324,9 +1379,9 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
         sp.nblocks = (sp.nitems + DM_THREADS * U - 1) / (DM_THREADS * U);
         if (sp.nblocks > 0x7fffffffu) return (int)hipErrorInvalidValue;
         if (sh.nt && ((u64)sizeof(S) << q) >= 128)   // the quad's columns in different 128 B lines
-            dm_sop_intile_kernel<S, U, 1><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
+            dm_sop_intile_kernel<S, U, 1, SKIP><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
         else
-            dm_sop_intile_kernel<S, U, 0><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
+            dm_sop_intile_kernel<S, U, 0, SKIP><<<(unsigned)sp.nblocks, DM_THREADS, sh.lds / U, st>>>((S *)rho, sp, op);
     } else {
         sp.m = q - L;
         const u64 h = nt >> 1;
@@ -1337,17 +1392,22 @@ int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op
         const unsigned nb = (unsigned)sp.nblocks;
         S *r = (S *)rho;
         if (sh.nt) {
-            if (lp == 2) dm_sop_tiles_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
-            else if (lp == 1) dm_sop_tiles_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
-            else dm_sop_tiles_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            if (lp == 2) dm_sop_tiles_kernel<S, 1, 4, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else if (lp == 1) dm_sop_tiles_kernel<S, 1, 2, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else dm_sop_tiles_kernel<S, 1, 1, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
         } else {
-            if (lp == 2) dm_sop_tiles_kernel<S, 0, 4><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
-            else if (lp == 1) dm_sop_tiles_kernel<S, 0, 2><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
-            else dm_sop_tiles_kernel<S, 0, 1><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            if (lp == 2) dm_sop_tiles_kernel<S, 0, 4, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else if (lp == 1) dm_sop_tiles_kernel<S, 0, 2, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
+            else dm_sop_tiles_kernel<S, 0, 1, SKIP><<<nb, DM_THREADS, lds, st>>>(r, sp, op);
         }
     }
     ORK_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename S>
+int launch_stream(hipStream_t st, void *rho, int L, int n, int q, const DmOp &op) {
+    return op.skip ? launch_stream_t<S, 1>(st, rho, L, n, q, op) : launch_stream_t<S, 0>(st, rho, L, n, q, op);
 }
 
 // streamed packed superoperator: column order and an 80 KB residency cap; each 32 x 32 block split
@@ -1363,8 +1423,8 @@ Shape g_pstream_shape = kPstreamShape;
 constexpr Shape kPstreamShape32 = {2, 1, 40960, 0, 1};
 Shape g_pstream_shape32 = kPstreamShape32;
 
-template <typename S>
-int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
+template <typename S, int SKIP>
+int launch_pstream_t(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     PackedStreamPlan pp;
     pp.dim = (u64)1 << n;
     pp.q = q;
@@ -1382,20 +1442,25 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
     // bit 6 (the wl field) with nontemporal accesses: only on the lines a workgroup owns alone
     const bool hybrid = sh.nt && sh.wl;
     if (hybrid) {
-        if (lp == 2) dm_sop_packed_kernel<S, 2, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else if (lp == 1) dm_sop_packed_kernel<S, 2, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else dm_sop_packed_kernel<S, 2, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        if (lp == 2) dm_sop_packed_kernel<S, 2, 4, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 2, 2, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 2, 1, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
     } else if (sh.nt) {
-        if (lp == 2) dm_sop_packed_kernel<S, 1, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else if (lp == 1) dm_sop_packed_kernel<S, 1, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else dm_sop_packed_kernel<S, 1, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        if (lp == 2) dm_sop_packed_kernel<S, 1, 4, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 1, 2, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 1, 1, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
     } else {
-        if (lp == 2) dm_sop_packed_kernel<S, 0, 4><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else if (lp == 1) dm_sop_packed_kernel<S, 0, 2><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
-        else dm_sop_packed_kernel<S, 0, 1><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        if (lp == 2) dm_sop_packed_kernel<S, 0, 4, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else if (lp == 1) dm_sop_packed_kernel<S, 0, 2, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
+        else dm_sop_packed_kernel<S, 0, 1, SKIP><<<nb, DM_THREADS, lds, st>>>(r, pp, op);
     }
     ORK_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename S>
+int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
+    return op.skip ? launch_pstream_t<S, 1>(st, rho, n, q, op) : launch_pstream_t<S, 0>(st, rho, n, q, op);
 }
 
 // whole-tile permutations: XCD runs of 32 workgroups, one workgroup per tile pair (splitting a pair over 2
@@ -1625,12 +1690,18 @@ extern "C" void ork_dm_reset_shapes(void) {
     }
     for (int i = 0; i < 4; ++i) g_stream_shape[i] = kStreamShape[i];
     g_stream_on = 1;
+    for (int i = 0; i < 3; ++i) g_skip_margin[i] = kSkipMargin[i];
     g_pstream_shape = kPstreamShape;
     g_pstream_shape32 = kPstreamShape32;
     g_permt_shape = kPermTileShape;
 }
 
 extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
+    if (which >= 23 && which <= 25) {   // skip margin of controlled gates (63: never skip)
+        if (flags < 0 || flags > 63 || lds_bytes != 0) return (int)hipErrorInvalidValue;
+        g_skip_margin[which - 23] = flags;
+        return 0;
+    }
     if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 127 || ((flags >> 3) & 3) > 2)
         return (int)hipErrorInvalidValue;
     // bit 5: XCD runs of 32 groups instead of contiguous ranges; bit 6: whole-line writes (phase kernels)
@@ -1688,7 +1759,7 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
             if (op.srt[b] < op.srt[a]) { int t = op.srt[a]; op.srt[a] = op.srt[b]; op.srt[b] = t; }
     for (int a = 0; a < 8; ++a) { op.pinv[a] = a; op.ph[a] = 0; }
     op.still = 0;
-    op.ctl = op.pu01 = op.pu10 = 0;
+    op.ctl = op.pu01 = op.pu10 = op.skip = 0;
     op.cmask = 0;
     if (opk == ORK_DM_MONO) {
         for (int a = 0; a < (1 << k); ++a) {
@@ -1709,24 +1780,33 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
     if (opk == ORK_DM_SOP2 && k != 2) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
     // a controlled gate with one target (cx, cy, ccx) runs through the streamed 1-qubit kernels on that
-    // target, which move every entry once: measured on 14 qubits this beats the whole-tile and in-tile
-    // permutations for cx / cy everywhere (3/4 of the entries move: 4.4-4.9 against 3.5-4.8 TB/s tiled,
-    // 3.6-4.2 against 3.0-3.8 packed) and for ccx unless every qubit is at or above the tile, where only
-    // its 7/16 moved entries are worth moving (whole tiles 3.5-4.4 against 2.6-2.7 TB/s)
+    // target.  Where every control qubit lies far enough above the bits of a 128 B line they move only
+    // the quads that change (cx 3/4 of the entries, ccx 7/16; 14 qubits: tiled cx 5.0-5.8 TB/s, packed
+    // cx 4.4-4.9, ccx 3.5-4.3, on those bytes), elsewhere every entry once, which still beats the
+    // in-tile and tile-group kernels.  With every qubit at or above the tile, whole tiles move faster on
+    // the tiled layout (ccx 4.6-5.6 against 3.6-4.7 TB/s, cx +2 %, complex64 cx +11 %); on the packed
+    // layout whole blocks lose to streaming (cx 3.9 against 4.3 TB/s) unless its quads cannot be skipped
+    // (shape 22 flag bit 6 sends packed cx / cy / ccx to whole blocks anyway)
     int tq, pu01, pu10;
     u64 cm;
-    if (opk == ORK_DM_MONO && g_stream_on && controlled_single_target(m, tq, pu01, pu10, cm) &&
-        !(m->k == 3 && perm_tiles_ok(layout, log_tile, n, m)) && !g_permt_shape.col &&
+    if (opk == ORK_DM_MONO && g_stream_on && !g_permt_shape.col && controlled_single_target(m, tq, pu01, pu10, cm) &&
         ((layout == ORK_PACKED && n >= 6) || (layout == ORK_TILED && n >= log_tile))) {
         DmOp c = op;
         c.ctl = 1;
         c.cmask = cm;
         c.pu01 = pu01;
         c.pu10 = pu10;
-        if (layout == ORK_PACKED)
-            return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, tq, c) : launch_pstream<double2>(st, rho, n, tq, c);
-        return prec == ORK_F32 ? launch_stream<float2>(st, rho, log_tile, n, tq, c)
-                               : launch_stream<double2>(st, rho, log_tile, n, tq, c);
+        const int line_bits = prec == ORK_F32 ? 4 : 3, lowc = cm ? __builtin_ctzll(cm) : 64;
+        c.skip = lowc >= line_bits + g_skip_margin[layout == ORK_TILED ? 1 : (m->k == 3 ? 2 : 0)];
+        const bool tiles = perm_tiles_ok(layout, log_tile, n, m) &&
+                           (layout == ORK_TILED || !c.skip || g_permt_shape.wl != 0);
+        if (!tiles) {
+            if (layout == ORK_PACKED)
+                return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, tq, c)
+                                       : launch_pstream<double2>(st, rho, n, tq, c);
+            return prec == ORK_F32 ? launch_stream<float2>(st, rho, log_tile, n, tq, c)
+                                   : launch_stream<double2>(st, rho, log_tile, n, tq, c);
+        }
     }
     if (opk == ORK_DM_MONO && perm_intile_ok(layout, log_tile, n, m)) {
         if (prec == ORK_F32)

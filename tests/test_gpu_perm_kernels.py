@@ -151,8 +151,9 @@ def test_dm_x_y_streamed_equal_tile_groups(eng, kind):
 @pytest.mark.parametrize("log_tile", [5, 3])
 def test_dm_tile_permutations_equal_tile_groups(kind, log_tile):
     """Gates whose qubits all lie at or above the tile (32 x 32 blocks for packed) move whole tiles
-    (dm_perm_tiles_kernel); hook 22 with flag bit 1 sends them to the tile-group kernel instead.  Both
-    move entries exactly: bit-identical, and equal to the oracle."""
+    (dm_perm_tiles_kernel; packed cx / cy / ccx stream unless hook 22 flag bit 6 is set); hook 22 with
+    flag bit 1 sends them to the tile-group kernel instead.  All move entries exactly: bit-identical,
+    and equal to the oracle."""
     dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
     api = dev.api
     orc = engines.oracle(log_tile)
@@ -163,16 +164,16 @@ def test_dm_tile_permutations_equal_tile_groups(kind, log_tile):
     ops += [("ccx", (a, b, c)) for a in hi for b in hi for c in hi if len({a, b, c}) == 3]
     rho = to_layout(random_rho(n, np.random.default_rng(90 + kind + log_tile)), kind, log_tile)
     outs = []
-    for off in (0, 2):
+    for flags in (0, 64, 2):
         assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
-        assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | off, 0) == 0
+        assert api.orkan_set_launch_shape(PERMT_HOOK, 32 | flags, 0) == 0
         st = dev.new(kind, n, rho)
         for name, qs in ops:
             dev.gate(st, name, *qs)
         outs.append(dev.read(st))
         dev.free(st)
     assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
-    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     so = orc.new(kind, n, rho)
     for name, qs in ops:
         orc.gate(so, name, *qs)
@@ -278,3 +279,53 @@ def test_dm_straddling_controlled_gates(kind, log_tile):
         orc.gate(so, name, *qs)
     assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
     orc.free(so)
+
+
+SKIP_HOOKS = (23, 24, 25)   # packed cx / cy, tiled, packed ccx
+
+
+@pytest.mark.parametrize("fp32", [False, True], ids=["c128", "c64"])
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+@pytest.mark.parametrize("log_tile", [5, 3])
+def test_dm_controlled_gates_skip_unchanged_quads(fp32, kind, log_tile):
+    """Where every control qubit lies above the bits of a 128 B line, the streamed kernels leave the
+    quads of cx / cy / ccx that keep their value unread and unwritten (hooks 23 / 24 / 25: the margin
+    above the line bits for packed cx / cy, tiled, packed ccx; 63 never).  Skipping, never skipping (ccx with every qubit at or above the tile then
+    moves whole tiles) and the permutation kernels (hook 8 off: whole-tile, in-tile and tile-group
+    kernels) give bit-identical results on every
+    ordered pair and a spread of triples, in complex128 and complex64 storage; complex128 also equals
+    the oracle."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(log_tile), log_tile)
+    api = dev.api
+    n = 8 if (kind == MIXED_PACKED or log_tile == 5) else 7
+    ops = [(g, (a, b)) for g in ("cx", "cy") for a in range(n) for b in range(n) if a != b]
+    ops += [("ccx", t) for t in ((0, 1, 2), (3, 4, 5), (5, 6, 7 % n), (6, 4, 3), (4, 5, 0), (n - 1, n - 2, n - 3),
+                                 (2, n - 1, 4), (n - 2, 3, n - 1))]
+    rho = to_layout(random_rho(n, np.random.default_rng(400 + kind + log_tile + 7 * fp32)), kind, log_tile)
+    outs = []
+    for mode in ("skip", "never", "groups"):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        for h in SKIP_HOOKS:   # skip wherever the controls lie above the line bits, or never
+            assert api.orkan_set_launch_shape(h, 63 if mode == "never" else 0, 0) == 0
+        if mode == "groups":
+            assert api.orkan_set_launch_shape(STREAM_HOOK, 0, 0) == 0
+        st = dev.new(kind, n, rho)
+        if fp32:
+            s32 = api.orkan_state_convert(C.byref(st), 1)
+            dev.free(st)
+            st = s32
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        out = np.empty(dev.length(st), np.complex64 if fp32 else np.complex128)
+        assert api.orkan_state_download(C.byref(st), out.ctypes.data) == 0
+        outs.append(out)
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    if not fp32:
+        orc = engines.oracle(log_tile)
+        so = orc.new(kind, n, rho)
+        for name, qs in ops:
+            orc.gate(so, name, *qs)
+        assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
+        orc.free(so)
