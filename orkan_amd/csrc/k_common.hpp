@@ -71,6 +71,21 @@ template <> struct Store<float2> {
         __builtin_nontemporal_store((float)v.y, &p->y);
     }
 };
+// raw storage loads, converted later: a complex64 load whose float -> double conversion follows at once inside a
+// condition makes the compiler wait for it there, one load in flight per thread; loading the raw values of a
+// whole batch without conditions first and converting afterwards keeps every load of the batch in flight
+template <int NT, typename S> __device__ __forceinline__ S ldr(const S *p) {
+    if constexpr (NT) {
+        S v;
+        v.x = __builtin_nontemporal_load(&p->x);
+        v.y = __builtin_nontemporal_load(&p->y);
+        return v;
+    } else {
+        return *p;
+    }
+}
+__device__ __forceinline__ cd to_cd(double2 v) { return v; }
+__device__ __forceinline__ cd to_cd(float2 v) { return cmk((double)v.x, (double)v.y); }
 template <typename S> __device__ __forceinline__ cd ldc(const S *p) { return Store<S>::ld(p); }
 template <typename S> __device__ __forceinline__ void stc(S *p, cd v) { Store<S>::st(p, v); }
 // lane exchange (within a wavefront) of one complex value as four 32-bit shuffles

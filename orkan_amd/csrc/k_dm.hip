@@ -942,6 +942,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
         const int php = (p.ph[pa] - p.ph[pb]) & 3;
         const bool pair = wp != w;
         cd vs[E], vp[E];
+        S rs[E], rp[E];   // raw values: converted once every load of the tile is in flight
         u64 os[E], op[E];
         bool ok[E];
         int ii[E], jj[E];
@@ -962,8 +963,15 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
                 perm_tile_entry<LAYOUT>(p, pr, pc, o, i2, j2, offp, stp);
             }
             op[e] = offp;
-            vs[e] = st ? ldx<NTA>(rho + off) : cmk(0.0, 0.0);
-            vp[e] = (st && pair) ? ldx<NTA>(rho + offp) : cmk(0.0, 0.0);
+            // loads without a condition (a conditional complex64 load waits for its own conversion):
+            // an entry that is not stored reads rho[0], a tile without a partner its own entry again
+            rs[e] = ldr<NTA>(rho + off);
+            rp[e] = ldr<NTA>(rho + ((st && pair) ? offp : off));
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            vs[e] = to_cd(rs[e]);
+            vp[e] = to_cd(rp[e]);
         }
         if (!trans) {
             // same in-tile positions: s takes p's entries and p takes s's (or s its own, phase only)

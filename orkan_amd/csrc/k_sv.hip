@@ -663,7 +663,25 @@ __global__ __launch_bounds__(THREADS) void sv_layer_kernel(S *__restrict__ psi, 
         }
         return;
     }
-    for (int e = threadIdx.x; e < T; e += THREADS) tile[lpad(e)] = ldx<NT>(psi + LIDX(e));
+    {
+        // batches of up to 8 loads per thread, all issued before the first LDS write (element 0's address
+        // stands in past the tile end, so that no load is conditional)
+        constexpr int PER = (1 << LT) / THREADS, BT = PER < 8 ? PER : 8;
+#pragma unroll 1
+        for (int u0 = 0; u0 < PER; u0 += BT) {
+            S raw[BT];
+#pragma unroll
+            for (int u = 0; u < BT; ++u) {
+                const int e = threadIdx.x + (u0 + u) * THREADS;
+                raw[u] = ldr<NT>(psi + LIDX(e < T ? e : 0));
+            }
+#pragma unroll
+            for (int u = 0; u < BT; ++u) {
+                const int e = threadIdx.x + (u0 + u) * THREADS;
+                if (e < T) tile[lpad(e)] = to_cd(raw[u]);
+            }
+        }
+    }
     __syncthreads();
     // up to three tile bits per pass: each thread holds the 8 amplitudes of one 3-bit group
     for (int b = a.b0; b < a.b1;) {

@@ -48,6 +48,28 @@ def test_layer_matches_per_qubit_application(eng, n):
     assert np.abs(got - _apply(psi, mats)).max() <= 1e-12
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_layer_tile_shapes(eng, variant):
+    """Every tile shape of tuning hook 17 (2^12 or 2^11 amplitudes, 256 or 512 threads) on sizes whose sweeps
+    take the register-resident path and the generic LDS path (a last sweep of 1-2 qubits)."""
+    api = eng.api
+    try:
+        assert api.orkan_set_launch_shape(17, variant, 0) == 0
+        for n in (13, 17, 22):
+            rng = np.random.default_rng(300 + 10 * variant + n)
+            psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+            psi /= np.linalg.norm(psi)
+            mats = _unitaries(rng, n)
+            st = eng.new(PURE, n, psi)
+            m = _colmajor(mats)
+            assert api.orkan_layer_1q(C.byref(st), m.ctypes.data) == 0
+            got = eng.read(st)
+            eng.free(st)
+            assert np.abs(got - _apply(psi, mats)).max() <= 1e-12, (variant, n)
+    finally:
+        assert api.orkan_set_launch_shape(14, 0, 0) == 0
+
+
 def test_layer_fp32_storage(eng):
     n = 16
     rng = np.random.default_rng(7)
