@@ -160,6 +160,12 @@ __device__ __forceinline__ cd lds_get(const cd *lds, const DmOp &op, int x, int 
 template <typename S, int LAYOUT, int OP, int HOIST, int NTA>
 __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rho, DmPlan p, DmOp op) {
     __shared__ cd lds[W_MAX * LDS_STRIDE];
+    // SOP2: the 16 x 16 superoperator staged in LDS once (every lane reads the same entry: broadcasts),
+    // instead of 256 dependent loads from device memory per 4 x 4 block and thread
+    __shared__ cd s2l[OP == ORK_DM_SOP2 ? 256 : 1];
+    if constexpr (OP == ORK_DM_SOP2) {
+        for (int i = threadIdx.x; i < 256; i += DM_THREADS) s2l[i] = op.s2[i];   // read after the load barrier
+    }
     constexpr int EPT = W_MAX * W_MAX / DM_THREADS;   // entries of a group per thread
     const int LB = p.LB, g = p.g, LW = p.LW;
     const int B = 1 << LB, W = 1 << LW, NT = 1 << g;
@@ -280,12 +286,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) v[a * 4 + c] = lds[xo[a] + yo[c]];
-#pragma unroll
+                // one output per iteration, its row of the superoperator read from LDS (a rolled loop: the
+                // 256 entries are not held in registers across blocks); the inputs v are all in registers
+#pragma unroll 1
                 for (int i = 0; i < 16; ++i) {
                     cd o = cmk(0.0, 0.0);
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) cfma(o, op.s2[i * 16 + j], v[j]);
-                    lds[xo[i >> 2] + yo[i & 3]] = o;
+                    for (int j = 0; j < 16; ++j) cfma(o, s2l[i * 16 + j], v[j]);
+                    lds[(x0 | (int)spread_bits(i >> 2, op.lbit, 2)) * LDS_STRIDE + (y0 | (int)spread_bits(i & 3, op.lbit, 2))] = o;
                 }
             }
             __syncthreads();
