@@ -150,7 +150,8 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * flags bit 0 keeps consecutive groups on one XCD, bit 1 walks the groups column by column, bit 2
  * (phase kernels 1, 4, 5 and streamed kernels 6, 7, 9, 15, 16, 18) uses nontemporal loads and stores,
  * bits 3-4 (kernels 7, 16 with 32 x 32 tiles; 9, 18) give log2 of the workgroups that share one tile quad
- * or packed block (0..2), bit 3 of kernels 6, 15 (with bit 2) selects the lane-exchange kernel for
+ * or packed block (0..2), and (phase kernels 1, 4, 5) log2 of the multiple of 64 B each thread has in
+ * flight (0..2; 0, the default: 1 x), bit 3 of kernels 6, 15 (with bit 2) selects the lane-exchange kernel for
  * in-tile targets whose quad columns share 128 B lines, bit 5 hands each XCD runs of 32 consecutive
  * groups instead of the contiguous ranges of bit 0, bit 6 (phase kernels 1, 4, 5) writes every 128 B
  * line holding a changed entry in full where the changed entries do not fill lines;

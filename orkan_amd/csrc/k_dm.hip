@@ -1105,7 +1105,8 @@ struct PhasePlan {
     u64 nchunks;   // workgroups of the launch
     int L;         // tiled: tiles of 2^L x 2^L (one zero-padded tile when n < L)
     int xcd;
-    int tile_exit; // tiled, one tile per workgroup: leave tiles without a changed entry at once
+    int tile_exit; // tiled, whole tiles per workgroup: leave at once when none holds a changed entry
+    int tpc;       // tiles per workgroup (tile_exit)
     u64 tchg;      // bit a * 8 + b: a tile whose gate bits above the tile are (a, b) holds a changed entry
 };
 
@@ -1151,11 +1152,18 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
     if (LAYOUT == ORK_PACKED) key = 0;
     if constexpr (MODE == 0 && LAYOUT == ORK_TILED) {
         if (pp.tile_exit) {
-            // the workgroup's tile w: its gate bits above the tile alone decide whether any entry changes
-            key = w;
-            tri_decode(w, P, Q);
-            const int at = gather_bits(P << pp.L, a.qs, a.k), bt = gather_bits(Q << pp.L, a.qs, a.k);
-            if (!((pp.tchg >> (at * 8 + bt)) & 1)) return;
+            // the workgroup's tiles w tpc ..: their gate bits above the tile alone decide whether any entry
+            // changes
+            bool any = false;
+            for (int u = 0; u < pp.tpc; ++u) {
+                const u64 t = w * (u64)pp.tpc + (u64)u;
+                if ((t << (2 * pp.L)) >= pp.len) break;
+                tri_decode(t, P, Q);
+                const int at = gather_bits(P << pp.L, a.qs, a.k), bt = gather_bits(Q << pp.L, a.qs, a.k);
+                any = any || ((pp.tchg >> (at * 8 + bt)) & 1);
+            }
+            if (!any) return;
+            if (pp.tpc == 1) key = w;   // P, Q hold the decode of tile w; else flat_rc decodes per tile
         }
     }
     cd f[E];
@@ -1598,18 +1606,21 @@ int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t 
     return 0;
 }
 
+// entries per thread: 64 B of each thread in flight, times 1, 2 or 4 (phase shape flags bits 3-4)
 template <typename S, int LAYOUT, int MODE, int NT, int WL>
 void launch_phase_t(hipStream_t st, unsigned grid, int lds, void *rho, const PhasePlan &pp, const PhaseArgs &a,
-                    const double2 *cs) {
-    constexpr int E = sizeof(S) == 8 ? 8 : 4;   // 64 B of each thread in flight
-    dm_phase_kernel<S, LAYOUT, MODE, NT, WL, E><<<grid, DM_THREADS, lds, st>>>((S *)rho, pp, a, cs);
+                    const double2 *cs, int em) {
+    constexpr int E = sizeof(S) == 8 ? 8 : 4;
+    if (em == 4) dm_phase_kernel<S, LAYOUT, MODE, NT, WL, 4 * E><<<grid, DM_THREADS, lds, st>>>((S *)rho, pp, a, cs);
+    else if (em == 2) dm_phase_kernel<S, LAYOUT, MODE, NT, WL, 2 * E><<<grid, DM_THREADS, lds, st>>>((S *)rho, pp, a, cs);
+    else dm_phase_kernel<S, LAYOUT, MODE, NT, WL, E><<<grid, DM_THREADS, lds, st>>>((S *)rho, pp, a, cs);
 }
 
 template <typename S, int MODE, int NT, int WL>
 void launch_phase_l(hipStream_t st, int layout, unsigned grid, int lds, void *rho, const PhasePlan &pp,
-                    const PhaseArgs &a, const double2 *cs) {
-    if (layout == ORK_PACKED) launch_phase_t<S, ORK_PACKED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs);
-    else launch_phase_t<S, ORK_TILED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs);
+                    const PhaseArgs &a, const double2 *cs, int em) {
+    if (layout == ORK_PACKED) launch_phase_t<S, ORK_PACKED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs, em);
+    else launch_phase_t<S, ORK_TILED, MODE, NT, WL>(st, grid, lds, rho, pp, a, cs, em);
 }
 
 template <int MODE>
@@ -1626,7 +1637,8 @@ int launch_phase(hipStream_t st, int prec, int layout, int L, int n, void *rho, 
     }
     Shape sh = g_phase_shape[layout == ORK_TILED ? 1 : 0];
     const u64 elem = prec == ORK_F32 ? 8 : 16;
-    const u64 per = (prec == ORK_F32 ? 8 : 4) * (u64)DM_THREADS;
+    const int em = 1 << sh.lp;
+    const u64 per = (prec == ORK_F32 ? 8 : 4) * (u64)em * DM_THREADS;
     pp.nchunks = (pp.len + per - 1) / per;
     if (pp.nchunks > 0x7fffffffu) return (int)hipErrorInvalidValue;
     pp.xcd = sh.xcd;
@@ -1645,8 +1657,11 @@ int launch_phase(hipStream_t st, int prec, int layout, int L, int n, void *rho, 
     // tiled, one tile per workgroup: a mask over the gate bits above the tile of the tiles that hold a
     // changed entry (any in-tile combination of the other gate bits not skipped)
     pp.tile_exit = 0;
+    pp.tpc = 1;
     pp.tchg = 0;
-    if (MODE == 0 && layout == ORK_TILED && n >= L && ((u64)1 << (2 * L)) == per) {
+    const u64 tsz = (u64)1 << (2 * L);
+    if (MODE == 0 && layout == ORK_TILED && n >= L && (tsz == per || (em > 1 && tsz * (u64)em == per))) {
+        pp.tpc = (int)(per / tsz);
         int hi = 0;   // gate arguments at or above the tile
         for (int j = 0; j < a.k; ++j) hi |= (a.qs[j] >= L) << j;
         for (int at = 0; at < (1 << a.k); ++at)
@@ -1666,19 +1681,19 @@ int launch_phase(hipStream_t st, int prec, int layout, int L, int n, void *rho, 
     const int lds = sh.lds;
     if (prec == ORK_F32) {
         if (sh.nt) {
-            if (wl) launch_phase_l<float2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs);
-            else launch_phase_l<float2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs);
+            if (wl) launch_phase_l<float2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs, em);
+            else launch_phase_l<float2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs, em);
         } else {
-            if (wl) launch_phase_l<float2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs);
-            else launch_phase_l<float2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs);
+            if (wl) launch_phase_l<float2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs, em);
+            else launch_phase_l<float2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs, em);
         }
     } else {
         if (sh.nt) {
-            if (wl) launch_phase_l<double2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs);
-            else launch_phase_l<double2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs);
+            if (wl) launch_phase_l<double2, MODE, 1, 1>(st, layout, grid, lds, rho, pp, a, cs, em);
+            else launch_phase_l<double2, MODE, 1, 0>(st, layout, grid, lds, rho, pp, a, cs, em);
         } else {
-            if (wl) launch_phase_l<double2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs);
-            else launch_phase_l<double2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs);
+            if (wl) launch_phase_l<double2, MODE, 0, 1>(st, layout, grid, lds, rho, pp, a, cs, em);
+            else launch_phase_l<double2, MODE, 0, 0>(st, layout, grid, lds, rho, pp, a, cs, em);
         }
     }
     ORK_LAUNCH_CHECK();
