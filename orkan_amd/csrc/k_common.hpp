@@ -84,6 +84,10 @@ template <int NT, typename S> __device__ __forceinline__ S ldr(const S *p) {
         return *p;
     }
 }
+// an empty register barrier: a conversion of v cannot move above it, so one placed after a batch of
+// conditional loads keeps the compiler from sinking each conversion into its load's condition (and waiting
+// for that load there)
+template <typename S> __device__ __forceinline__ void hold(S &v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
 __device__ __forceinline__ cd to_cd(double2 v) { return v; }
 __device__ __forceinline__ cd to_cd(float2 v) { return cmk((double)v.x, (double)v.y); }
 template <typename S> __device__ __forceinline__ cd ldc(const S *p) { return Store<S>::ld(p); }

@@ -1039,6 +1039,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
     tri_decode(w, TR, TC);
     const bool dg = TR == TC;
     cd v[4];
+    S raw[4];
     u64 off[4];
     bool ld[4];
     int ii[4], jj[4];
@@ -1063,7 +1064,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
         // mirror of its upper-half partner, which changes exactly when it does)
         const bool chg = p.P[i] != i || p.P[j] != j || ((p.ph[i] - p.ph[j]) & 3);
         ld[e] = stored && chg;
-        v[e] = ld[e] ? ldx<NTA>(rho + off[e]) : cmk(0.0, 0.0);
+        raw[e].x = 0;
+        raw[e].y = 0;
+        if (ld[e]) raw[e] = ldr<NTA>(rho + off[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if constexpr (sizeof(S) == 8) hold(raw[e]);   // complex64: every load issued before the first conversion
+        v[e] = to_cd(raw[e]);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -1096,8 +1104,18 @@ struct PhaseArgs {
     int k;
     int qs[3];
     unsigned skip[8];   // bit b of skip[a]: entries with row index a, column index b unchanged
+    u64 skipm;          // the same as one mask, bit a * 8 + b (tested without a memory load)
     cd f[64];           // f[a * 8 + b] = F[a] conj(F[b])
 };
+
+// gate bits of index x (bit j <- qubit qj) from qubits held in registers: the kernels read qs once
+// instead of once per entry from the kernel arguments
+__device__ __forceinline__ int gate_bits(u64 x, int k, int q0, int q1, int q2) {
+    int v = (int)((x >> q0) & 1);
+    if (k > 1) v |= (int)((x >> q1) & 1) << 1;
+    if (k > 2) v |= (int)((x >> q2) & 1) << 2;
+    return v;
+}
 
 struct PhasePlan {
     u64 dim;       // D = 2^n
@@ -1148,6 +1166,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
     const u64 w = remap_group(blockIdx.x, pp.nchunks, pp.xcd);
     const u64 x0 = w * (u64)(E * DM_THREADS) + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    const int gk = a.k, q0 = a.qs[0], q1 = a.qs[1], q2 = a.qs[2];
+    const u64 skipm = a.skipm;
     u64 key = ~(u64)0, P = 0, Q = 0;
     if (LAYOUT == ORK_PACKED) key = 0;
     if constexpr (MODE == 0 && LAYOUT == ORK_TILED) {
@@ -1159,7 +1179,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
                 const u64 t = w * (u64)pp.tpc + (u64)u;
                 if ((t << (2 * pp.L)) >= pp.len) break;
                 tri_decode(t, P, Q);
-                const int at = gather_bits(P << pp.L, a.qs, a.k), bt = gather_bits(Q << pp.L, a.qs, a.k);
+                const int at = gate_bits(P << pp.L, gk, q0, q1, q2), bt = gate_bits(Q << pp.L, gk, q0, q1, q2);
                 any = any || ((pp.tchg >> (at * 8 + bt)) & 1);
             }
             if (!any) return;
@@ -1177,8 +1197,8 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
         ok = ok && r < pp.dim && c < pp.dim && r != c;   // padding of a one-tile state; the diagonal
         f[j] = cmk(1.0, 0.0);
         if constexpr (MODE == 0) {
-            const int ar = gather_bits(r, a.qs, a.k), ac = gather_bits(c, a.qs, a.k);
-            ok = ok && !(a.skip[ar] >> ac & 1);
+            const int ar = gate_bits(r, gk, q0, q1, q2), ac = gate_bits(c, gk, q0, q1, q2);
+            ok = ok && !((skipm >> (ar * 8 + ac)) & 1);
             if (ok) f[j] = a.f[ar * 8 + ac];
         } else if (ok) {
             // e^{-i(d_r - d_c)t} = (cr - i sr)(cc + i sc)   (src/circ/circ_packed.c:54-60)
@@ -1193,9 +1213,19 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
             act[j] = ok;
         }
     }
+    S raw[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        raw[j].x = 0;
+        raw[j].y = 0;
+        if (act[j]) raw[j] = ldr<NT>(rho + x0 + (u64)j * DM_THREADS);
+    }
     cd v[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = act[j] ? ldx<NT>(rho + x0 + (u64)j * DM_THREADS) : cmk(0.0, 0.0);
+    for (int j = 0; j < E; ++j) {
+        if constexpr (sizeof(S) == 8) hold(raw[j]);   // complex64: every load issued before the first conversion
+        v[j] = to_cd(raw[j]);
+    }
 #pragma unroll
     for (int j = 0; j < E; ++j) {
         if (!act[j]) continue;
@@ -1646,10 +1676,11 @@ int launch_phase(hipStream_t st, int prec, int layout, int L, int n, void *rho, 
     int wl = 0;
     if (MODE == 0) {
         // tiled: the lowest gate qubit decides whether changed entries fill 128 B lines (tile rows are
-        // line-aligned); packed: column runs start anywhere in a line, so any line may be partial
+        // line-aligned); runs of 64 B, aligned halves of lines, write as well without the rest of the line;
+        // packed: column runs start anywhere in a line, so any line may be partial
         int lo = 7;     // past a 128 B line for either storage
         for (int i = 0; i < a.k; ++i) lo = a.qs[i] < lo ? a.qs[i] : lo;
-        if ((elem << lo) < 128 || layout == ORK_PACKED) {
+        if ((elem << lo) < 64 || layout == ORK_PACKED) {
             wl = sh.wl;
             if (!wl) sh.nt = 0;   // partial-line writes stay cached
         }
@@ -1885,6 +1916,10 @@ extern "C" int ork_dm_phase(void *stream, int prec, int layout, int log_tile, vo
             if (re == 1.0 && im == 0.0) a.skip[x] |= 1u << y;
         }
     }
+    a.skipm = 0;
+    for (int x = 0; x < 8; ++x)
+        for (int y = 0; y < 8; ++y)
+            if ((a.skip[x] >> y) & 1) a.skipm |= (u64)1 << (x * 8 + y);
     return launch_phase<0>((hipStream_t)stream, prec, layout, log_tile, n, rho, a, nullptr);
 }
 
@@ -1899,6 +1934,8 @@ extern "C" int ork_dm_exp_diag(void *stream, int prec, int layout, int log_tile,
     ORK_LAUNCH_CHECK();
     PhaseArgs a;
     a.k = 0;
+    a.qs[0] = a.qs[1] = a.qs[2] = 0;   // per-index phases (mode 1): no gate qubits, no skip table
+    a.skipm = 0;
     return launch_phase<1>(st, prec, layout, log_tile, n, rho, a, (const double2 *)phase);
 }
 
