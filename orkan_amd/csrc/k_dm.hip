@@ -1025,9 +1025,21 @@ struct PermInPlan {
     u64 ntiles, D;
     int L;              // tiled: 5; packed: 5 (blocks)
     int xcd;
-    signed char P[32];  // in-tile index -> source in-tile index
-    signed char ph[32]; // in-tile index -> phase exponent (mod 4) of its source
+    int k, q0, q1, q2;  // gate arguments (in-tile bits)
+    unsigned pinv_bits; // bits 3a..3a+2: gate bits a take the amplitude with gate bits pinv[a] ...
+    unsigned ph_bits;   // bits 2s, 2s+1: ... times i^ph[s] of that source s
 };
+
+// in-tile index i -> in-tile index of its source, and the phase exponent of that source, computed from the
+// packed gate tables (a table lookup per entry in the kernel arguments waited for each load)
+__device__ __forceinline__ int intile_src(const PermInPlan &p, int i, int &ph) {
+    int a = (i >> p.q0) & 1;
+    if (p.k > 1) a |= ((i >> p.q1) & 1) << 1;
+    if (p.k > 2) a |= ((i >> p.q2) & 1) << 2;
+    const int sa = (int)((p.pinv_bits >> (3 * a)) & 7), d = a ^ sa;
+    ph = (int)((p.ph_bits >> (2 * sa)) & 3);
+    return i ^ ((d & 1) << p.q0) ^ (((d >> 1) & 1) << p.q1) ^ (((d >> 2) & 1) << p.q2);
+}
 
 template <typename S, int LAYOUT, int NTA>
 __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restrict__ rho, PermInPlan p) {
@@ -1041,6 +1053,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
     cd v[4];
     S raw[4];
     u64 off[4];
+    int si[4], sj[4], pd[4];   // sources of row and column, phase difference
     bool ld[4];
     int ii[4], jj[4];
 #pragma unroll
@@ -1062,7 +1075,11 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
         const bool stored = LAYOUT == ORK_TILED || !dg || i >= j;
         // changed: moves or takes a phase (in a packed diagonal block, an entry is also needed as the
         // mirror of its upper-half partner, which changes exactly when it does)
-        const bool chg = p.P[i] != i || p.P[j] != j || ((p.ph[i] - p.ph[j]) & 3);
+        int phi, phj;
+        si[e] = intile_src(p, i, phi);
+        sj[e] = intile_src(p, j, phj);
+        pd[e] = (phi - phj) & 3;
+        const bool chg = si[e] != i || sj[e] != j || pd[e];
         ld[e] = stored && chg;
         raw[e].x = 0;
         raw[e].y = 0;
@@ -1084,7 +1101,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
     for (int e = 0; e < 4; ++e) {
         if (!ld[e]) continue;
         const int i = ii[e], j = jj[e];
-        stx<NTA>(rho + off[e], cipow(t[p.P[i] * LDS_STRIDE + p.P[j]], p.ph[i] - p.ph[j]));
+        stx<NTA>(rho + off[e], cipow(t[si[e] * LDS_STRIDE + sj[e]], pd[e]));
     }
 }
 
@@ -1583,15 +1600,14 @@ int launch_perm_intile(hipStream_t st, void *rho, int n, const ork_mono_t *m) {
     int pinv[8];
     for (int a = 0; a < 8; ++a) pinv[a] = a;
     for (int a = 0; a < (1 << m->k); ++a) pinv[m->perm[a]] = a;
-    for (int x = 0; x < 32; ++x) {
-        int a = 0;
-        for (int j = 0; j < m->k; ++j) a |= ((x >> m->qs[j]) & 1) << j;
-        const int sa = pinv[a];
-        int y = x;
-        for (int j = 0; j < m->k; ++j)
-            if (((a ^ sa) >> j) & 1) y ^= 1 << m->qs[j];
-        p.P[x] = (signed char)y;
-        p.ph[x] = (signed char)(m->ph[sa] & 3);
+    p.k = m->k;
+    p.q0 = m->qs[0];
+    p.q1 = m->k > 1 ? m->qs[1] : 0;
+    p.q2 = m->k > 2 ? m->qs[2] : 0;
+    p.pinv_bits = p.ph_bits = 0;
+    for (int a = 0; a < 8; ++a) {
+        p.pinv_bits |= (unsigned)pinv[a] << (3 * a);
+        p.ph_bits |= (unsigned)(a < (1 << m->k) ? (m->ph[a] & 3) : 0) << (2 * a);
     }
     const Shape sh = g_permt_shape;
     p.xcd = sh.xcd;
