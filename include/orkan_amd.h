@@ -143,7 +143,11 @@ double orkan_event_ms(int a, int b);      /* elapsed ms between two recorded eve
  * to the tile-group kernel instead, bit 2 nontemporal accesses (tiled layout), bits 3-4 log4 of the
  * consecutive tiles each workgroup walks (0..2), bit 6 moves packed cx / cy / ccx with every qubit at
  * or above the 32 x 32 blocks as whole blocks (else they stream where their unchanged entries can be
- * left alone); 23 / 24 / 25: cx and cy on packed states / controlled
+ * left alone), bit 7 (kernel 22 only) sends swap and the other permutations whose lowest qubit lies
+ * below the tile and highest at or above it to the tile-group kernel (else they move virtual tiles of
+ * 2^q x 2^q entries, q the lowest qubit, where q >= 3 for complex128 and q >= 4 for complex64, with
+ * bit 2's nontemporal accesses for complex64 only);
+ * 23 / 24 / 25: cx and cy on packed states / controlled
  * gates on tiled states / ccx on packed states leave the entries they do not change unread and
  * unwritten when their lowest control qubit is at least the bits of a 128 B line (3 complex128, 4
  * complex64) plus flags (0..63, 63: never; defaults 2 / 0 / 0; lds_bytes must be 0);

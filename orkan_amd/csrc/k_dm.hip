@@ -887,8 +887,10 @@ struct PermTilePlan {
     int pinv[8];  // gate bits a take the tile whose gate bits are pinv[a] ...
     int ph[8];    // ... times i^(ph[pinv a] - ph[pinv b])
     int xcd;
-    int tpw;      // consecutive tiles per workgroup
+    int tpw;      // consecutive tiles (slot sets of virtual tiles) per workgroup
     u64 nwg;      // workgroups of the launch
+    int V;        // virtual tile log (dm_perm_subtiles_kernel; pos[] then counts from V)
+    unsigned pinv_bits, ph_bits;   // pinv[a] at bits 3a..3a+2, ph[s] at bits 2s, 2s+1
 };
 
 __device__ __forceinline__ u64 perm_src_tile(const PermTilePlan &p, u64 t, int &a_out) {
@@ -1010,6 +1012,222 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
             for (int e = 0; e < E; ++e)
                 if (ok[e]) stx<NTA>(rho + op[e], cipow(cconj(ts[jj[e] * LDS_STRIDE + ii[e]]), php));
             __syncthreads();   // before the next tile's LDS writes
+        }
+    }
+}
+
+// ---- permutations of virtual tiles smaller than the storage tile -----------------------------------
+// A gate whose lowest qubit V lies below the tile and whose highest at or above it (swap(4, 5) with
+// 32 x 32 tiles) maps virtual tiles of 2^V x 2^V entries onto virtual tiles: the rule above with tile
+// indices r >> V.  The rows of a virtual tile are runs of 2^V entries (V >= 3: whole 128 B lines for
+// complex128, V >= 4 for complex64), where the tile-group kernel moved 8-entry runs.  On the tiled layout a
+// virtual tile lies in the storage tile (TR >> d, TC >> d), d = L - V; the enumerated tiles are the lower
+// virtual tiles (TR >= TC), every source is read from a lower one, and the upper virtual tiles of diagonal
+// storage tiles (stored, never read here) are rewritten as the conjugate transpose of their new lower
+// mirror by the owner of that mirror.  Each workgroup iteration takes 64 consecutive candidate tiles; its
+// first wave lists those that own a pair or change alone (3/8 of them for swap), and slot sets of 1024
+// entries walk the list: 2^(10 - 2V) virtual tiles per set, slot e of a thread holding entry
+// (tid + 256 e) mod 2^2V of listed tile (tid + 256 e) >> 2V, so that every slot has work (walked tile by
+// tile, 5/8 of the slots stood idle: 3.5 TB/s for swap(4, 5), 14 qubits, against 4.7 listed and 2.45 in
+// the tile-group kernel).  (Emulated on the CPU against U rho U^dagger for every gate of the set, V from 1
+// to the tile, both layouts, every qubit order: every position written at most once, no workgroup reading
+// what another writes.)
+__device__ __forceinline__ u64 sub_src_tile(const PermTilePlan &p, u64 t, int &s_out) {
+    int a = (int)((t >> p.pos[0]) & 1);
+    if (p.k > 1) a |= (int)((t >> p.pos[1]) & 1) << 1;
+    if (p.k > 2) a |= (int)((t >> p.pos[2]) & 1) << 2;
+    const int s = (int)((p.pinv_bits >> (3 * a)) & 7), x = a ^ s;
+    s_out = s;
+    return t ^ ((u64)(x & 1) << p.pos[0]) ^ ((u64)((x >> 1) & 1) << p.pos[1]) ^ ((u64)((x >> 2) & 1) << p.pos[2]);
+}
+__device__ __forceinline__ int sub_ph(const PermTilePlan &p, int s) { return (int)((p.ph_bits >> (2 * s)) & 3); }
+
+// entry o (memory order) of virtual tile (TR, TC): in-tile position, offset, stored
+template <int LAYOUT>
+__device__ __forceinline__ void sub_tile_entry(const PermTilePlan &p, u64 TR, u64 TC, int o, u64 &off, bool &st) {
+    const int V = p.V;
+    if constexpr (LAYOUT == ORK_TILED) {
+        const int d = p.L - V;
+        const u64 m = ((u64)1 << d) - 1;
+        off = tile_base(TR >> d, TC >> d, p.L) + (((((TR & m) << V) + (u64)(o >> V))) << p.L) + ((TC & m) << V) +
+              (u64)(o & ((1 << V) - 1));
+        st = true;
+    } else {
+        const u64 r = (TR << V) + (u64)(o & ((1 << V) - 1)), c = (TC << V) + (u64)(o >> V);
+        st = r >= c;
+        off = st ? col_base(p.D, c) + (r - c) : 0;
+    }
+}
+
+#ifndef ORK_SUB_COLUMNS
+#define ORK_SUB_COLUMNS 1
+#endif
+// candidate c -> virtual tile (TR, TC): on the packed layout down the columns of virtual tiles (a virtual
+// tile's column runs share their end lines with the tiles above and below), on the tiled layout along rows
+template <int LAYOUT>
+__device__ __forceinline__ void sub_tile_at(const PermTilePlan &p, u64 c, u64 &TR, u64 &TC) {
+    if (LAYOUT == ORK_PACKED && ORK_SUB_COLUMNS) {
+        u64 rr, cc;
+        tri_decode(p.ntiles - 1 - c, rr, cc);
+        const u64 nt = p.D >> p.V;
+        TR = nt - 1 - cc;
+        TC = nt - 1 - rr;
+    } else {
+        tri_decode(c, TR, TC);
+    }
+}
+
+// does virtual tile (TR, TC) own its pair (or change alone)?  Ownership follows the row-major index
+// w = TR (TR + 1) / 2 + TC whatever the walk; its partner (pr, pc) and the phase exponent ps of its source
+__device__ __forceinline__ bool sub_owner(const PermTilePlan &p, u64 TR, u64 TC, u64 &pr, u64 &pc, bool &tr, int &ps) {
+    const u64 w = TR * (TR + 1) / 2 + TC;
+    int sa, sb;
+    const u64 PR = sub_src_tile(p, TR, sa), PC = sub_src_tile(p, TC, sb);
+    tr = PR < PC;
+    pr = tr ? PC : PR;
+    pc = tr ? PR : PC;
+    const u64 wp = pr * (pr + 1) / 2 + pc;
+    ps = (sub_ph(p, sa) - sub_ph(p, sb)) & 3;
+    return wp >= w && (wp != w || tr || ps != 0);
+}
+
+template <typename S, int LAYOUT, int NTA>
+__global__ __launch_bounds__(DM_THREADS) void dm_perm_subtiles_kernel(S *__restrict__ rho, PermTilePlan p) {
+    __shared__ cd ts[W_MAX * LDS_STRIDE];
+    // listed tiles of an iteration (offsets from its first candidate), two buffers: the first wave writes the
+    // next iteration's list while the other waves may still read this one's count
+    __shared__ int act[2][64];
+    __shared__ int nact[2];
+    constexpr int E = W_MAX * W_MAX / DM_THREADS;
+    const int V = p.V, T = 1 << V, tid = threadIdx.x;
+    const int lsub = LW_MAX - V;          // virtual tiles per LDS row of tiles: 2^lsub
+    const int d = LAYOUT == ORK_TILED ? p.L - V : 0;
+    const u64 dm = ((u64)1 << d) - 1;
+    // offset from an entry of a lower virtual tile of a diagonal storage tile to the same in-tile position of
+    // the upper one, per unit of (TC & dm) - (TR & dm)
+    const long long mstep = LAYOUT == ORK_TILED ? ((1ll << (V + p.L)) - (1ll << V)) : 0;
+    const u64 g = remap_group(blockIdx.x, p.nwg, p.xcd);
+    for (int tw = 0; tw < p.tpw; ++tw) {
+        const u64 base = (g * (u64)p.tpw + (u64)tw) * 64;
+        if (base >= p.ntiles) break;
+        if (tid < 64) {   // the first wave lists the tiles with work
+            const u64 c = base + (u64)tid;
+            bool a = false;
+            if (c < p.ntiles) {
+                u64 TR, TC, pr, pc;
+                bool tr;
+                int ps;
+                sub_tile_at<LAYOUT>(p, c, TR, TC);
+                a = sub_owner(p, TR, TC, pr, pc, tr, ps);
+            }
+            const u64 bal = __ballot(a);
+            if (a) act[tw & 1][__popcll(bal & ((1ull << tid) - 1))] = tid;
+            if (tid == 0) nact[tw & 1] = __popcll(bal);
+        }
+        __syncthreads();
+        const int cnt = nact[tw & 1];
+        for (int c0 = 0; c0 < cnt; c0 += 1 << (2 * lsub)) {
+        S rs[E], rp[E];
+        u64 os[E], op[E];
+        int ii[E], jj[E], lo[E], ms[E], mp[E], phs[E], php[E];
+        bool ok[E], pair[E], trans[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int oa = tid + e * DM_THREADS, t = oa >> (2 * V), o = oa & (T * T - 1);
+            if constexpr (LAYOUT == ORK_TILED) { ii[e] = o >> V; jj[e] = o & (T - 1); }
+            else { jj[e] = o >> V; ii[e] = o & (T - 1); }
+            lo[e] = (t >> lsub) * T * LDS_STRIDE + (t & ((1 << lsub) - 1)) * T;
+            ok[e] = pair[e] = trans[e] = false;
+            ms[e] = mp[e] = phs[e] = php[e] = 0;
+            u64 off = 0, offp = 0;
+            if (c0 + t < cnt) {
+                u64 TR, TC, pr, pc;
+                bool tr;
+                int ps;
+                sub_tile_at<LAYOUT>(p, base + (u64)act[tw & 1][c0 + t], TR, TC);
+                (void)sub_owner(p, TR, TC, pr, pc, tr, ps);
+                {
+                    bool st;
+                    sub_tile_entry<LAYOUT>(p, TR, TC, o, off, st);
+                    if (st) {
+                        ok[e] = true;
+                        trans[e] = tr;
+                        pair[e] = pr != TR || pc != TC;
+                        phs[e] = ps;
+                        if (LAYOUT == ORK_TILED && TR != TC && (TR >> d) == (TC >> d))
+                            ms[e] = (int)(TC & dm) - (int)(TR & dm);
+                        if (pair[e]) {
+                            bool stp;
+                            sub_tile_entry<LAYOUT>(p, pr, pc, o, offp, stp);
+                            int pa, pb;
+                            (void)sub_src_tile(p, pr, pa);
+                            (void)sub_src_tile(p, pc, pb);
+                            php[e] = (sub_ph(p, pa) - sub_ph(p, pb)) & 3;
+                            if (LAYOUT == ORK_TILED && pr != pc && (pr >> d) == (pc >> d))
+                                mp[e] = (int)(pc & dm) - (int)(pr & dm);
+                        }
+                    }
+                }
+            }
+            os[e] = off;
+            op[e] = offp;
+            // loads without a condition: a slot with nothing to do reads rho[0], a tile without a partner
+            // its own entry again
+            rs[e] = ldr<NTA>(rho + off);
+            rp[e] = ldr<NTA>(rho + (pair[e] ? offp : off));
+        }
+        cd vs[E], vp[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            vs[e] = to_cd(rs[e]);
+            vp[e] = to_cd(rp[e]);
+        }
+        // new s (i, j) = i^phs src(i, j), src = old p (old s without a partner), or i^phs conj(src(j, i))
+        // for a transposed pair; the upper mirror of a new tile X takes conj(X(j, i)).  Outputs without a
+        // transpose go straight from registers.
+        bool needa = false, needb = false;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (!ok[e]) continue;
+            const cd src = sel(pair[e], vp[e], vs[e]);
+            if (!trans[e]) stx<NTA>(rho + os[e], cipow(src, phs[e]));
+            else if (ms[e]) stx<NTA>(rho + os[e] + ms[e] * mstep, cipow(src, -phs[e]));
+            if (pair[e]) {
+                if (!trans[e]) stx<NTA>(rho + op[e], cipow(vs[e], php[e]));
+                else if (mp[e]) stx<NTA>(rho + op[e] + mp[e] * mstep, cipow(vs[e], -php[e]));
+            }
+            needa |= trans[e] || ms[e] != 0;
+            needb |= pair[e] && (trans[e] || mp[e] != 0);
+        }
+        // transposed outputs through LDS, one virtual tile per T x T block (each barrier also keeps the
+        // previous round's LDS reads ahead of the next round's writes)
+        if (__syncthreads_or(needa)) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (ok[e] && (trans[e] || ms[e])) ts[lo[e] + ii[e] * LDS_STRIDE + jj[e]] = sel(pair[e], vp[e], vs[e]);
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (!ok[e] || !(trans[e] || ms[e])) continue;
+                const cd x = cconj(ts[lo[e] + jj[e] * LDS_STRIDE + ii[e]]);
+                if (trans[e]) stx<NTA>(rho + os[e], cipow(x, phs[e]));
+                else stx<NTA>(rho + os[e] + ms[e] * mstep, cipow(x, -phs[e]));
+            }
+        }
+        if (__syncthreads_or(needb)) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (ok[e] && pair[e] && (trans[e] || mp[e])) ts[lo[e] + ii[e] * LDS_STRIDE + jj[e]] = vs[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (!ok[e] || !pair[e] || !(trans[e] || mp[e])) continue;
+                const cd x = cconj(ts[lo[e] + jj[e] * LDS_STRIDE + ii[e]]);
+                if (trans[e]) stx<NTA>(rho + op[e], cipow(x, php[e]));
+                else stx<NTA>(rho + op[e] + mp[e] * mstep, cipow(x, -php[e]));
+            }
+        }
+        // the next slot set writes LDS only after its first barrier, which follows these reads
         }
     }
 }
@@ -1345,7 +1563,8 @@ DmPlan make_plan(int layout, int L, int n, const int *qs, int k) {
 // bytes are fetched); tiled: row order with the remap.
 // nt: nontemporal loads and stores; lp: log2 of the workgroups sharing one tile quad (cross-tile kernel)
 // wl: whole-line writes of the phase kernels (flags bit 6)
-struct Shape { int xcd, col, lds, nt, lp, wl; };
+// nosub (whole-tile permutations only): straddling gates to the tile-group kernel instead of virtual tiles
+struct Shape { int xcd, col, lds, nt, lp, wl, nosub; };
 constexpr Shape kGroupShape[2] = {{1, 1, 0, 0}, {1, 0, 0, 1}};   // [packed, tiled]
 // phase kernels (cz, exp_diag, ...): flat streams of E entries per thread, XCD runs of 32 workgroups,
 // nontemporal accesses, whole-line writes where the changed entries do not fill lines
@@ -1540,17 +1759,23 @@ int launch_pstream(hipStream_t st, void *rho, int n, int q, const DmOp &op) {
 constexpr Shape kPermTileShape = {2, 0, 0, 1, 0, 0};
 Shape g_permt_shape = kPermTileShape;
 
-// can the gate run through dm_perm_tiles_kernel?  An involution whose qubits all lie at or above the
-// tile (log2 T: the storage tile, or 32 x 32 blocks for packed)
-bool perm_tiles_ok(int layout, int L, int n, const ork_mono_t *m) {
-    if (g_permt_shape.col) return false;
+// the virtual tile log on which the whole-tile kernels run the gate, or -1: an involution whose qubits all
+// lie at or above the tile (log2 T: the storage tile, or 32 x 32 blocks for packed) runs on whole tiles
+// (V = T); with `sub`, one whose lowest qubit lies below the tile runs on virtual tiles of that log where
+// their rows fill 128 B lines (V >= 3 complex128, V >= 4 complex64)
+int perm_tiles_v(int layout, int L, int n, int prec, const ork_mono_t *m, bool sub) {
+    if (g_permt_shape.col) return -1;
     const int T = layout == ORK_TILED ? L : 5;
-    if (n <= T) return false;
-    for (int j = 0; j < m->k; ++j)
-        if (m->qs[j] < T) return false;
+    if (n <= T) return -1;
+    int lo = T;
+    for (int j = 0; j < m->k; ++j) lo = m->qs[j] < lo ? m->qs[j] : lo;
     for (int a = 0; a < (1 << m->k); ++a)
-        if (m->perm[m->perm[a]] != a) return false;
-    return true;
+        if (m->perm[m->perm[a]] != a) return -1;
+    if (lo == T) return T;
+    return (sub && !g_permt_shape.nosub && lo >= (prec == ORK_F32 ? 4 : 3)) ? lo : -1;
+}
+bool perm_tiles_ok(int layout, int L, int n, const ork_mono_t *m) {
+    return perm_tiles_v(layout, L, n, ORK_F64, m, false) >= 0;
 }
 
 // Is the gate U on one target qubit controlled by the others (cx, cy, ccx)?  Then tq = the target,
@@ -1621,31 +1846,49 @@ int launch_perm_intile(hipStream_t st, void *rho, int n, const ork_mono_t *m) {
 }
 
 template <typename S, int LAYOUT>
-int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t *m) {
+int launch_perm_tiles(hipStream_t st, void *rho, int n, int L, const ork_mono_t *m, int V) {
     PermTilePlan p;
     p.D = (u64)1 << n;
     p.L = LAYOUT == ORK_TILED ? L : 5;
-    const u64 nt = p.D >> p.L;
+    p.V = V;
+    const u64 nt = p.D >> V;
     p.ntiles = nt * (nt + 1) / 2;
-    if (p.ntiles > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    if (p.ntiles > 0x7fffffffu || V > p.L || V > LW_MAX) return (int)hipErrorInvalidValue;
     p.k = m->k;
-    for (int j = 0; j < 3; ++j) p.pos[j] = j < m->k ? m->qs[j] - p.L : 0;
+    for (int j = 0; j < 3; ++j) p.pos[j] = j < m->k ? m->qs[j] - V : 0;
     for (int a = 0; a < 8; ++a) { p.pinv[a] = a; p.ph[a] = 0; }
     for (int a = 0; a < (1 << m->k); ++a) {
         p.pinv[m->perm[a]] = a;
         p.ph[a] = m->ph[a] & 3;
     }
+    p.pinv_bits = p.ph_bits = 0;
+    for (int a = 0; a < 8; ++a) {
+        p.pinv_bits |= (unsigned)p.pinv[a] << (3 * a);
+        p.ph_bits |= (unsigned)p.ph[a] << (2 * a);
+    }
     const Shape sh = g_permt_shape;
     p.xcd = sh.xcd;
     p.tpw = 1 << (2 * sh.lp);   // flags bits 3-4: log4 of the tiles per workgroup
-    p.nwg = (p.ntiles + p.tpw - 1) / p.tpw;
     S *r = (S *)rho;
+    const int stat = (int)sizeof(cd) * W_MAX * LDS_STRIDE;
+    const int lds = sh.lds > stat ? sh.lds - stat : 0;
+    if (V < p.L) {
+        // virtual tiles below the storage tile: 64 candidate tiles per workgroup iteration
+        const u64 per = ((u64)p.tpw) * 64;
+        p.nwg = (p.ntiles + per - 1) / per;
+        // nontemporal accesses for complex64 only (complex128: 4-9 % slower with them, 14 qubits)
+        const bool ntm = LAYOUT == ORK_TILED && sh.nt && sizeof(S) == 8 &&
+                         (p.ntiles << (2 * V)) * sizeof(S) >= ork_nt_min_bytes();
+        if (ntm) dm_perm_subtiles_kernel<S, LAYOUT, 1><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
+        else dm_perm_subtiles_kernel<S, LAYOUT, 0><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
+        ORK_LAUNCH_CHECK();
+        return 0;
+    }
+    p.nwg = (p.ntiles + p.tpw - 1) / p.tpw;
     // nontemporal accesses (flags bit 2) on the tiled layout, whose tiles are whole lines, on states of at
     // least the nontemporal minimum; the packed layout's block columns share lines with their neighbours
     const bool ntm = LAYOUT == ORK_TILED && sh.nt && (p.ntiles << (2 * p.L)) * sizeof(S) >= ork_nt_min_bytes() &&
                      ((u64)sizeof(S) << p.L) >= 128;
-    const int stat = (int)sizeof(cd) * W_MAX * LDS_STRIDE;
-    const int lds = sh.lds > stat ? sh.lds - stat : 0;
     if (ntm) dm_perm_tiles_kernel<S, LAYOUT, 1><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
     else dm_perm_tiles_kernel<S, LAYOUT, 0><<<(unsigned)p.nwg, DM_THREADS, lds, st>>>(r, p);
     ORK_LAUNCH_CHECK();
@@ -1780,11 +2023,12 @@ extern "C" int ork_dm_set_shape(int which, int flags, int lds_bytes) {
         g_skip_margin[which - 23] = flags;
         return 0;
     }
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > 127 || ((flags >> 3) & 3) > 2)
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024 || flags < 0 || flags > (which == 22 ? 255 : 127) ||
+        ((flags >> 3) & 3) > 2)
         return (int)hipErrorInvalidValue;
     // bit 5: XCD runs of 32 groups instead of contiguous ranges; bit 6: whole-line writes (phase kernels)
     const Shape sh{(flags & 32) ? 2 : (flags & 1), (flags >> 1) & 1, lds_bytes, (flags >> 2) & 1, (flags >> 3) & 3,
-                   (flags >> 6) & 1};
+                   (flags >> 6) & 1, (flags >> 7) & 1};
     switch (which) {
     case 0: g_group_shape[0] = g_group_shape[1] = sh; break;
     case 1: g_phase_shape[0] = g_phase_shape[1] = sh; break;
@@ -1893,12 +2137,13 @@ extern "C" int ork_dm_group(void *stream, int prec, int layout, int log_tile, vo
         return layout == ORK_PACKED ? launch_perm_intile<double2, ORK_PACKED>(st, rho, n, m)
                                     : launch_perm_intile<double2, ORK_TILED>(st, rho, n, m);
     }
-    if (opk == ORK_DM_MONO && perm_tiles_ok(layout, log_tile, n, m)) {
+    const int pv = opk == ORK_DM_MONO ? perm_tiles_v(layout, log_tile, n, prec, m, true) : -1;
+    if (pv >= 0) {
         if (prec == ORK_F32)
-            return layout == ORK_PACKED ? launch_perm_tiles<float2, ORK_PACKED>(st, rho, n, log_tile, m)
-                                        : launch_perm_tiles<float2, ORK_TILED>(st, rho, n, log_tile, m);
-        return layout == ORK_PACKED ? launch_perm_tiles<double2, ORK_PACKED>(st, rho, n, log_tile, m)
-                                    : launch_perm_tiles<double2, ORK_TILED>(st, rho, n, log_tile, m);
+            return layout == ORK_PACKED ? launch_perm_tiles<float2, ORK_PACKED>(st, rho, n, log_tile, m, pv)
+                                        : launch_perm_tiles<float2, ORK_TILED>(st, rho, n, log_tile, m, pv);
+        return layout == ORK_PACKED ? launch_perm_tiles<double2, ORK_PACKED>(st, rho, n, log_tile, m, pv)
+                                    : launch_perm_tiles<double2, ORK_TILED>(st, rho, n, log_tile, m, pv);
     }
     if (layout == ORK_PACKED && opk == ORK_DM_SOP && g_stream_on && n >= 6)
         return prec == ORK_F32 ? launch_pstream<float2>(st, rho, n, m->qs[0], op)

@@ -5,7 +5,8 @@ launch shape of tuning hook 21, with nontemporal accesses forced on through hook
 x and y run through the streamed superoperator kernels (their superoperators hold 0, +-1, +-i only),
 cx / cy / ccx through the same kernels on their target as a controlled permutation of each quad
 (ccx with every qubit at or above the tile excepted), swap and that ccx through the whole-tile or
-in-tile permutation kernels where their qubits allow, everything else through the tile-group kernel,
+in-tile permutation kernels where their qubits allow, swap across the tile edge through virtual tiles
+below the tile, everything else through the tile-group kernel,
 which leaves unchanged entries unread and unwritten.  Each path is checked against the oracle and
 bit for bit against the tile-group kernel (hooks 8 and 22 switch the others off)."""
 import ctypes as C
@@ -324,6 +325,54 @@ def test_dm_controlled_gates_skip_unchanged_quads(fp32, kind, log_tile):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     if not fp32:
         orc = engines.oracle(log_tile)
+        so = orc.new(kind, n, rho)
+        for name, qs in ops:
+            orc.gate(so, name, *qs)
+        assert float(np.abs(outs[0] - orc.read(so)).max()) <= 1e-12
+        orc.free(so)
+
+
+@pytest.mark.parametrize("fp32", [False, True], ids=["c128", "c64"])
+@pytest.mark.parametrize("kind", [MIXED_TILED, MIXED_PACKED], ids=["tiled", "packed"])
+def test_dm_subtile_permutations(fp32, kind):
+    """A gate whose lowest qubit V lies below the 32 x 32 tile and whose highest at or above it moves
+    virtual tiles of 2^V x 2^V entries (V 3 or 4 for complex128, 4 for complex64): swap by default, and cx /
+    cy / ccx once the streamed kernels are switched off (hook 8), cy with its phases.  On the tiled layout
+    the upper virtual tiles of diagonal storage tiles are rewritten from their lower mirrors.  The results
+    equal, bit for bit, those of the tile-group kernel (hook 22 flag bit 7 sends these gates there), with
+    nontemporal accesses and without, with one and with 16 iterations per workgroup (hook 22 bits 3-4), and
+    in complex128 the oracle."""
+    dev = engines.DeviceEngine(orkan_amd.lib_path(5), 5)
+    api = dev.api
+    n = 10
+    ops = []
+    for a in ((4,) if fp32 else (3, 4)):
+        for b in (5, 6, 9):
+            c = 8 if b != 8 else 7
+            ops += [("swap_gate", (a, b)), ("swap_gate", (b, a)), ("cx", (a, b)), ("cx", (b, a)), ("cy", (b, a)),
+                    ("cy", (a, b)), ("ccx", (b, a, c)), ("ccx", (a, c, b)), ("ccx", (c, b, a))]
+    rho = to_layout(random_rho(n, np.random.default_rng(600 + kind + 7 * fp32)), kind, 5)
+    outs = []
+    # virtual tiles with and without nontemporal accesses, 16 iterations per workgroup; tile groups
+    for flags in (32 | 4, 32, 32 | 4 | 16, 32 | 4 | 128):
+        assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(STREAM_HOOK, 0, 0) == 0
+        assert api.orkan_set_launch_shape(PERMT_HOOK, flags, 0) == 0
+        st = dev.new(kind, n, rho)
+        if fp32:
+            s32 = api.orkan_state_convert(C.byref(st), 1)
+            dev.free(st)
+            st = s32
+        for name, qs in ops:
+            dev.gate(st, name, *qs)
+        out = np.empty(dev.length(st), np.complex64 if fp32 else np.complex128)
+        assert api.orkan_state_download(C.byref(st), out.ctypes.data) == 0
+        outs.append(out)
+        dev.free(st)
+    assert api.orkan_set_launch_shape(RESET_HOOK, 0, 0) == 0
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    if not fp32:
+        orc = engines.oracle(5)
         so = orc.new(kind, n, rho)
         for name, qs in ops:
             orc.gate(so, name, *qs)

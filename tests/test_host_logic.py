@@ -167,8 +167,8 @@ def test_launch_shape_hook_ranges_without_device():
           (16, 4 | 8, 49152), (17, 3, 0), (17, 0, 0), (18, 32 | 16 | 2, 81920), (19, 2, 49152), (19, 0, 0), (19, 4 | 2, 32768), (20, 0, 53248),
           (12, 16 | 2 | 4, 53248), (12, 0, 0), (13, 15, 0), (13, 0, 1), (14, 0, 0)]
     ok += [(23, 0, 0), (24, 63, 0), (23, 2, 0), (25, 1, 0), (9, 32 | 16 | 2, 81920), (7, 32 | 4, 49152), (5, 64 | 32 | 4, 0), (4, 64, 65536), (22, 2, 0), (22, 33, 49152), (2, 64 | 3, 0),
-           (21, 1, 0), (21, 255, 81920)]
-    bad = [(-1, 0, 0), (21, 0, 0), (26, 0, 0), (25, 64, 0), (23, -1, 0), (24, 64, 0), (23, 0, 4096), (21, 256, 0), (21, 4, 0), (20, 0, -1), (18, 24, 0), (19, 3, 0), (19, 7, 0), (19, 8, 0), (19, 0, -1), (17, 4, 0), (9, 128, 0), (7, 24, 0), (0, -1, 0), (0, 128, 0), (7, 24, 0), (0, 0, -1), (0, 0, 160 * 1024 + 1),
+           (21, 1, 0), (21, 255, 81920), (22, 128 | 36, 0), (22, 128, 0)]
+    bad = [(-1, 0, 0), (21, 0, 0), (22, 256, 0), (22, 128 | 24, 0), (3, 128, 0), (26, 0, 0), (25, 64, 0), (23, -1, 0), (24, 64, 0), (23, 0, 4096), (21, 256, 0), (21, 4, 0), (20, 0, -1), (18, 24, 0), (19, 3, 0), (19, 7, 0), (19, 8, 0), (19, 0, -1), (17, 4, 0), (9, 128, 0), (7, 24, 0), (0, -1, 0), (0, 128, 0), (7, 24, 0), (0, 0, -1), (0, 0, 160 * 1024 + 1),
            (10, 0, -1), (10, 10, 0), (10, 26, 0), (10, 32, 0), (10, -1, 0), (11, 3, 0), (11, -1, 0), (12, 32, 0), (12, 16 | 12, 0), (13, 16, 0), (13, 0, -1)]
     try:
         for k, f, l in ok:
