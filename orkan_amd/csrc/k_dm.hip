@@ -271,25 +271,33 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
             __syncthreads();
         } else if constexpr (OP == ORK_DM_SOP2) {
             // 2-qubit superoperator on each 4x4 block: out[(a,b)] = sum_(c,d) S[(a,b),(c,d)] in[(c,d)]
-            const int LW2 = LW - 2;
-            for (int b = tid; b < (total >> 4); b += DM_THREADS) {
-                const int x0 = (int)ins0k((u64)(b >> LW2), op.srt, 2);
-                const int y0 = (int)ins0k((u64)(b & ((1 << LW2) - 1)), op.srt, 2);
+            // four threads per block (at most 64 blocks: one item per thread), each computing 4 of its 16
+            // outputs; the block's inputs are all read before the barrier, the outputs written after it.
+            // (One thread per block left three of the four waves idle through 256 complex FMAs.)
+            const int LW2 = LW - 2, nb = total >> 4;
+            const bool act = tid < 4 * nb;
+            const int b = act ? tid % nb : 0, qd = act ? tid / nb : 0;   // block, quarter of its outputs
+            const int x0 = (int)ins0k((u64)(b >> LW2), op.srt, 2);
+            const int y0 = (int)ins0k((u64)(b & ((1 << LW2) - 1)), op.srt, 2);
+            cd v[16];
+            if (act) {
                 int xo[4], yo[4];
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     xo[a] = (x0 | (int)spread_bits(a, op.lbit, 2)) * LDS_STRIDE;
                     yo[a] = y0 | (int)spread_bits(a, op.lbit, 2);
                 }
-                cd v[16];
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) v[a * 4 + c] = lds[xo[a] + yo[c]];
-                // one output per iteration, its row of the superoperator read from LDS (a rolled loop: the
-                // 256 entries are not held in registers across blocks); the inputs v are all in registers
+            }
+            __syncthreads();
+            if (act) {
+                // output rows 4 qd .. 4 qd + 3 of the superoperator (with 64 blocks a whole wave shares them,
+                // so their LDS reads are broadcasts); a rolled loop keeps the rows out of registers
 #pragma unroll 1
-                for (int i = 0; i < 16; ++i) {
+                for (int i = 4 * qd; i < 4 * qd + 4; ++i) {
                     cd o = cmk(0.0, 0.0);
 #pragma unroll
                     for (int j = 0; j < 16; ++j) cfma(o, s2l[i * 16 + j], v[j]);
