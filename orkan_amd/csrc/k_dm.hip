@@ -768,6 +768,10 @@ template <int NT, typename S> __device__ __forceinline__ cd ldh(const S *p, bool
     if constexpr (NT == 2) return own ? ldx<1>(p) : ldx<0>(p);
     else return ldx<NT>(p);
 }
+template <int NT, typename S> __device__ __forceinline__ S ldhr(const S *p, bool own) {
+    if constexpr (NT == 2) return own ? ldr<1>(p) : ldr<0>(p);
+    else return ldr<NT>(p);
+}
 template <int NT, typename S> __device__ __forceinline__ void sth(S *p, cd v, bool own) {
     if constexpr (NT == 2) {
         if (own) stx<1>(p, v);
@@ -806,9 +810,12 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
             if (mirror(a, b)) mv[h] = ldx<NT == 1>(rho + col_base(D, ins0(a, q)) + ((ins0(b, q) | m) - ins0(a, q)));
         }
     }
-    // all quads of a thread in flight: t = tid + 256 h
+    // all quads of a thread in flight: t = tid + 256 h.  Complex64 values are loaded raw for every quad
+    // (a quad left alone reads rho[0]) and converted after the batch: a load inside the quad's condition
+    // had the compiler wait for its conversion there, one quad's loads in flight per thread
     {
         cd v[QPT][4];
+        S raw[QPT][4];
         S *p0[QPT], *p1[QPT], *p2[QPT], *p3[QPT];
         int kind[QPT];   // 0 skip, 1 direct, 2 mirror, 3 diagonal quad
         int own[QPT];    // bit j: the line of p_j is the workgroup's alone (NT 2)
@@ -832,12 +839,28 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_packed_kernel(S *__restrict
                              (int)packed_line_owned<S>(D, q, A, r1, c1) << 3;
                     if (kind[h] == 1) own[h] |= (int)packed_line_owned<S>(D, q, A, r0, c1) << 1;
                 }
-                v[h][0] = ldh<NT>(p0[h], own[h] & 1);
-                v[h][2] = ldh<NT>(p2[h], (own[h] >> 2) & 1);
-                v[h][3] = ldh<NT>(p3[h], (own[h] >> 3) & 1);
-                if (kind[h] == 1) v[h][1] = ldh<NT>(p1[h], (own[h] >> 1) & 1);
+            }
+            if constexpr (sizeof(S) != 8) {   // complex128: loaded only where used, no conversion to wait for
+#pragma unroll
+                for (int j = 0; j < 4; ++j) raw[h][j].x = raw[h][j].y = 0.0;
+            }
+            if (sizeof(S) == 8 || kind[h]) {
+                raw[h][0] = ldhr<NT>(p0[h], own[h] & 1);
+                raw[h][2] = ldhr<NT>(p2[h], (own[h] >> 2) & 1);
+                raw[h][3] = ldhr<NT>(p3[h], (own[h] >> 3) & 1);
+                if (sizeof(S) == 8 || kind[h] == 1) raw[h][1] = ldhr<NT>(p1[h], (own[h] >> 1) & 1);
             }
         }
+        if constexpr (sizeof(S) == 8) {
+#pragma unroll
+            for (int h = 0; h < QPT; ++h)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) hold(raw[h][j]);
+        }
+#pragma unroll
+        for (int h = 0; h < QPT; ++h)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[h][j] = to_cd(raw[h][j]);
         if (has_mirror) {
 #pragma unroll
             for (int h = 0; h < QPT; ++h) {
