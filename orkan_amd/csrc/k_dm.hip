@@ -127,11 +127,23 @@ __device__ __forceinline__ bool tile_stored(const DmPlan &p, u64 TR, u64 TC) {
     else return (TR >> (p.L - p.LB)) >= (TC >> (p.L - p.LB));
 }
 
+// Permutations: per LDS coordinate x (one row or column index of the group), a table entry built once per
+// workgroup: bits 0-7 the source coordinate (x with its gate bits a replaced by pinv[a]), bits 8-9 the
+// phase exponent ph[pinv a] of that source, bits 12-14 the gate bits a.  (Computing these per entry read
+// pinv[] and ph[] from the kernel arguments with a per-lane index, a memory load waited for per entry.)
+__device__ __forceinline__ int mono_entry(const DmOp &op, int x) {
+    const int k = op.k;
+    const u64 mask = spread_bits((1 << k) - 1, op.lbit, k);
+    const int a = gather_bits((u64)x, op.lbit, k), ai = op.pinv[a];
+    const int xs = (int)(((u64)x & ~mask) | spread_bits(ai, op.lbit, k));
+    return xs | (op.ph[ai] & 3) << 8 | a << 12;
+}
+
 // does LDS entry (x, y) change?  Only a permutation leaves entries as they are.
 template <int OP>
-__device__ __forceinline__ bool moved(const DmOp &op, int x, int y) {
+__device__ __forceinline__ bool moved(const DmOp &op, const int *mt, int x, int y) {
     if constexpr (OP == ORK_DM_MONO) {
-        const int a = gather_bits((u64)x, op.lbit, op.k), b = gather_bits((u64)y, op.lbit, op.k);
+        const int a = (mt[x] >> 12) & 7, b = (mt[y] >> 12) & 7;
         return !((op.still >> (a * 8 + b)) & 1);
     } else {
         return true;
@@ -141,14 +153,10 @@ __device__ __forceinline__ bool moved(const DmOp &op, int x, int y) {
 // LDS entry (x, y) of the updated group; permutations with phases are applied
 // here as a gather: rho'(perm a, perm b) = i^(ph[a] - ph[b]) rho(a, b)
 template <int OP>
-__device__ __forceinline__ cd lds_get(const cd *lds, const DmOp &op, int x, int y) {
+__device__ __forceinline__ cd lds_get(const cd *lds, const int *mt, int x, int y) {
     if constexpr (OP == ORK_DM_MONO) {
-        const int k = op.k;
-        const u64 mask = spread_bits((1 << k) - 1, op.lbit, k);
-        const int ai = op.pinv[gather_bits((u64)x, op.lbit, k)], bi = op.pinv[gather_bits((u64)y, op.lbit, k)];
-        const int xs = (int)(((u64)x & ~mask) | spread_bits(ai, op.lbit, k));
-        const int ys = (int)(((u64)y & ~mask) | spread_bits(bi, op.lbit, k));
-        return cipow(lds[xs * LDS_STRIDE + ys], op.ph[ai] - op.ph[bi]);
+        const int ex = mt[x], ey = mt[y];
+        return cipow(lds[(ex & 0xff) * LDS_STRIDE + (ey & 0xff)], ((ex >> 8) & 3) - ((ey >> 8) & 3));
     } else {
         return lds[x * LDS_STRIDE + y];
     }
@@ -163,14 +171,25 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
     // SOP2: the 16 x 16 superoperator staged in LDS once (every lane reads the same entry: broadcasts),
     // instead of 256 dependent loads from device memory per 4 x 4 block and thread
     __shared__ cd s2l[OP == ORK_DM_SOP2 ? 256 : 1];
-    if constexpr (OP == ORK_DM_SOP2) {
-        for (int i = threadIdx.x; i < 256; i += DM_THREADS) s2l[i] = op.s2[i];   // read after the load barrier
-    }
     constexpr int EPT = W_MAX * W_MAX / DM_THREADS;   // entries of a group per thread
     const int LB = p.LB, g = p.g, LW = p.LW;
     const int B = 1 << LB, W = 1 << LW, NT = 1 << g;
     const int total = W * W;
     const int tid = threadIdx.x;
+    // per-workgroup tables: the tile-index bits of each group tile i (the gate qubits at or above LB spread
+    // to their positions), and the permutation entries of each LDS coordinate
+    __shared__ u64 gsp[8];
+    __shared__ int mt[OP == ORK_DM_MONO ? W_MAX : 1];
+    if (tid < 8) gsp[tid] = tid < NT ? spread_bits(tid, p.gpos, g) : 0;
+    if constexpr (OP == ORK_DM_MONO) {
+        if (tid < W) mt[tid] = mono_entry(op, tid);
+    }
+    if (OP == ORK_DM_MONO || g) __syncthreads();
+    // SOP2: the 16 x 16 superoperator, read only after the first group's load barrier (staged after the
+    // barrier above: waiting there for these loads cost each workgroup, most of which take one group, ~2 %)
+    if constexpr (OP == ORK_DM_SOP2) {
+        for (int i = threadIdx.x; i < 256; i += DM_THREADS) s2l[i] = op.s2[i];
+    }
 
     for (u64 w0 = blockIdx.x; w0 < p.ngroups; w0 += gridDim.x) {
         u64 R, C;
@@ -193,7 +212,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                 if (t >= total) continue;
                 const int ti = t >> (2 * LB), e = t & (B * B - 1);
                 const int i = ti >> g, j = ti & (NT - 1);
-                const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
+                const u64 TR = TRb | (g ? gsp[i] : 0), TC = TCb | (g ? gsp[j] : 0);
                 const int sa = e >> LB, sb = e & (B - 1);                  // slow, fast
                 const int sr = (LAYOUT == ORK_TILED) ? sa : sb;            // row-major vs column-major
                 const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
@@ -201,14 +220,14 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
                     if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
                     lx[u] = i * B + sr;
                     ly[u] = j * B + sc;
-                    if (!moved<OP>(op, lx[u], ly[u])) continue;
+                    if (!moved<OP>(op, mt, lx[u], ly[u])) continue;
                     lk[u] = (LAYOUT == ORK_PACKED && TR == TC && sr > sc) ? 2 : 1;
                     lv[u] = ldx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
                 } else {
                     // stored mirror tile (TC, TR): its entry (sr, sc) is conj of ours at (sc, sr)
                     lx[u] = i * B + sc;
                     ly[u] = j * B + sr;
-                    if (!moved<OP>(op, lx[u], ly[u])) continue;
+                    if (!moved<OP>(op, mt, lx[u], ly[u])) continue;
                     lk[u] = 3;
                     lv[u] = ldx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
                 }
@@ -223,19 +242,19 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
             for (int t = tid; t < total; t += DM_THREADS) {
                 const int ti = t >> (2 * LB), e = t & (B * B - 1);
                 const int i = ti >> g, j = ti & (NT - 1);
-                const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
+                const u64 TR = TRb | (g ? gsp[i] : 0), TC = TCb | (g ? gsp[j] : 0);
                 const int sa = e >> LB, sb = e & (B - 1);
                 const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
                 const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
                 if (tile_stored<LAYOUT>(p, TR, TC)) {
                     if (LAYOUT == ORK_PACKED && TR == TC && sr < sc) continue;
-                    if (!moved<OP>(op, i * B + sr, j * B + sc)) continue;
+                    if (!moved<OP>(op, mt, i * B + sr, j * B + sc)) continue;
                     const cd v = ldx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc));
                     lds[(i * B + sr) * LDS_STRIDE + j * B + sc] = v;
                     if (LAYOUT == ORK_PACKED && TR == TC && sr > sc)
                         lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
                 } else {
-                    if (!moved<OP>(op, i * B + sc, j * B + sr)) continue;
+                    if (!moved<OP>(op, mt, i * B + sc, j * B + sr)) continue;
                     const cd v = ldx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc));
                     lds[(i * B + sc) * LDS_STRIDE + j * B + sr] = cconj(v);
                 }
@@ -354,15 +373,15 @@ __global__ __launch_bounds__(DM_THREADS) void dm_group_kernel(S *__restrict__ rh
         for (int t = tid; t < total; t += DM_THREADS) {
             const int ti = t >> (2 * LB), e = t & (B * B - 1);
             const int i = ti >> g, j = ti & (NT - 1);
-            const u64 TR = TRb | spread_bits(i, p.gpos, g), TC = TCb | spread_bits(j, p.gpos, g);
+            const u64 TR = TRb | (g ? gsp[i] : 0), TC = TCb | (g ? gsp[j] : 0);
             const int sa = e >> LB, sb = e & (B - 1);
             const int sr = (LAYOUT == ORK_TILED) ? sa : sb;
             const int sc = (LAYOUT == ORK_TILED) ? sb : sa;
             const bool direct = tile_stored<LAYOUT>(p, TR, TC) && !(LAYOUT == ORK_PACKED && TR == TC && sr < sc) &&
-                                moved<OP>(op, i * B + sr, j * B + sc);
-            const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR) && moved<OP>(op, i * B + sc, j * B + sr);
-            if (direct) stx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, op, i * B + sr, j * B + sc));
-            if (mirror) stx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, op, i * B + sc, j * B + sr)));
+                                moved<OP>(op, mt, i * B + sr, j * B + sc);
+            const bool mirror = !diag_group && tile_stored<LAYOUT>(p, TC, TR) && moved<OP>(op, mt, i * B + sc, j * B + sr);
+            if (direct) stx<NTA>(rho + elem_off<LAYOUT>(p, TR * B + sr, TC * B + sc), lds_get<OP>(lds, mt, i * B + sr, j * B + sc));
+            if (mirror) stx<NTA>(rho + elem_off<LAYOUT>(p, TC * B + sr, TR * B + sc), cconj(lds_get<OP>(lds, mt, i * B + sc, j * B + sr)));
         }
         __syncthreads();
     }
