@@ -267,6 +267,9 @@ struct PermLaneArgs {
     int qs[3];
     int pinv[8];   // gate bits a take the amplitude whose gate bits are pinv[a]
     int ph[8];     // ... times i^ph[pinv[a]]
+    // the same tables packed (pinv[a] at bits 3a..3a+2, ph[s] at bits 2s, 2s+1): read per lane from the
+    // arrays, each lookup was a memory load waited for
+    unsigned pinv_bits, ph_bits;
 };
 
 template <typename S, int U, int XCD, int NT, int WL>
@@ -276,20 +279,30 @@ __global__ __launch_bounds__(SV_THREADS) void sv_perm_lane_kernel(S *__restrict_
     const int lane = threadIdx.x & 63;
     int a = 0;
     for (int j = 0; j < m.k; ++j) a |= ((lane >> m.qs[j]) & 1) << j;
-    const int s = m.pinv[a];
+    const int s = (int)((m.pinv_bits >> (3 * a)) & 7);
     int d = 0;
     for (int j = 0; j < m.k; ++j) d |= (((a ^ s) >> j) & 1) << m.qs[j];
-    const int src = lane ^ d, ph = m.ph[s] & 3;
+    const int src = lane ^ d, ph = (int)((m.ph_bits >> (2 * s)) & 3);
     const bool chg = (s != a) || ph != 0;
     const unsigned long long cm = __ballot(chg);   // depends on the lane only, the same for every chunk
     const bool line = ((cm >> (lane & ~(G - 1))) & ((1ull << G) - 1)) != 0;
     const bool wr = WL ? line : chg;
-    cd v[U];
+    // raw loads without a condition (a lane outside the changed lines reads psi[0], a value no lane uses),
+    // converted after the batch: a complex64 load inside the condition waited for its own conversion, one
+    // load in flight per thread
+    S r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const u64 x = base + (u64)u * SV_THREADS;
-        v[u] = (line && x < dim) ? ldx<NT>(psi + x) : cmk(0.0, 0.0);
+        r[u] = ldr<NT>(psi + ((line && x < dim) ? x : 0));
     }
+    if constexpr (sizeof(S) == 8) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) hold(r[u]);
+    }
+    cd v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = to_cd(r[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const u64 x = base + (u64)u * SV_THREADS;
@@ -804,10 +817,11 @@ struct PermShape {
     int x2_on, x2_lds;         // complex64: 16 B elements when no gate qubit is bit 0
 };
 // measured on 2^28 amplitudes (cx on every neighbour pair): complex128 lane kernel 4 amplitudes per thread
-// under a 40 KB cap (low pairs 3.24 TB/s against 2.66 through the per-group kernel), complex64 8 per
-// thread uncapped (2.92 against 2.63); whole-line writes (without them 2.0-2.2); the paired complex64
-// kernel under a 52 KB cap (5.9-6.2 TB/s on pairs from (4, 5) up, against 3.3 with 8 B accesses)
-constexpr PermShape kPerm = {1, 0, 4, 8, 40960, 0, 1, 1, 53248};
+// under a 40 KB cap (low pairs 3.24 TB/s against 2.66 through the per-group kernel), complex64 4 per
+// thread uncapped (low pairs 3.11-3.16, cx(4, 3) 5.2-5.3; 8 per thread 2.9 and 5.0 once the loads were
+// batched, 2.9 and 3.8 before); whole-line writes (without them 2.0-2.2); the paired complex64 kernel
+// under a 52 KB cap (5.9-6.2 TB/s on pairs from (4, 5) up, against 3.3 with 8 B accesses)
+constexpr PermShape kPerm = {1, 0, 4, 4, 40960, 0, 1, 1, 53248};
 static PermShape g_perm = kPerm;
 
 template <typename S, int U>
@@ -900,6 +914,11 @@ extern "C" int ork_sv_mono(void *stream, int prec, void *psi, int n, const ork_m
         for (int x = 0; x < (1 << a.k); ++x) {
             a.pinv[mono->perm[x]] = x;
             a.ph[x] = mono->ph[x] & 3;
+        }
+        a.pinv_bits = a.ph_bits = 0;
+        for (int x = 0; x < 8; ++x) {
+            a.pinv_bits |= (unsigned)a.pinv[x] << (3 * x);
+            a.ph_bits |= (unsigned)a.ph[x] << (2 * x);
         }
         const u64 dim = (u64)1 << n;
         const int u = prec == ORK_F32 ? g_perm.lane_u32 : g_perm.lane_u;
