@@ -462,6 +462,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
     S *t[U];
     int a00[U];
     cd v[U][4];
+    S raw[U][4];   // complex64: loaded without a condition, converted after the batch
     bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -476,12 +477,31 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_kernel(S *__restrict
             tri_decode(Q >> (2 * hl), TR, TC);
             ok[u] = !quad_still<SKIP>(op, (TR << L) | (u64)r0, (TC << L) | (u64)c0);
         }
-        if (ok[u]) {
+        if constexpr (sizeof(S) == 8) {
+            // a quad without work reads rho[0]: a complex64 load inside the condition waited for its own
+            // conversion there, one quad in flight per thread
+            S *tp = ok[u] ? t[u] : rho;
+            const int aa = ok[u] ? a00[u] : 0, mm = ok[u] ? mb : 0;
+            raw[u][0] = ldr<NT>(tp + aa);
+            raw[u][1] = ldr<NT>(tp + (aa | mm));
+            raw[u][2] = ldr<NT>(tp + (aa | (mm << L)));
+            raw[u][3] = ldr<NT>(tp + (aa | (mm << L) | mm));
+        } else if (ok[u]) {
             v[u][0] = ldx<NT>(t[u] + a00[u]);
             v[u][1] = ldx<NT>(t[u] + (a00[u] | mb));
             v[u][2] = ldx<NT>(t[u] + (a00[u] | (mb << L)));
             v[u][3] = ldx<NT>(t[u] + (a00[u] | (mb << L) | mb));
         }
+    }
+    if constexpr (sizeof(S) == 8) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hold(raw[u][j]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[u][j] = to_cd(raw[u][j]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -518,6 +538,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__res
     const int per_tile_log = 2 * L - 1;   // T / 2 row pairs x T columns per tile
     S *pa[U], *pb[U];
     cd a[U], b[U];
+    S ra[U], rb[U];   // complex64: loaded without a condition, converted after the batch
     bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -535,9 +556,24 @@ __global__ __launch_bounds__(DM_THREADS) void dm_sop_intile_lane_kernel(S *__res
             tri_decode(Q >> per_tile_log, TR, TC);
             ok[u] = !quad_still<SKIP>(op, (TR << L) | (u64)r0, (TC << L) | (u64)(c & ~mb));
         }
-        if (ok[u]) {
+        if constexpr (sizeof(S) == 8) {   // a pair without work reads rho[0] (as in dm_sop_intile_kernel)
+            ra[u] = ldr<NT>(ok[u] ? pa[u] : rho);
+            rb[u] = ldr<NT>(ok[u] ? pb[u] : rho);
+        } else if (ok[u]) {
             a[u] = ldx<NT>(pa[u]);
             b[u] = ldx<NT>(pb[u]);
+        }
+    }
+    if constexpr (sizeof(S) == 8) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            hold(ra[u]);
+            hold(rb[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a[u] = to_cd(ra[u]);
+            b[u] = to_cd(rb[u]);
         }
     }
     // bit q of the column is bit q of the lane (blocks start on multiples of 256 items)

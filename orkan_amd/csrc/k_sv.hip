@@ -157,13 +157,37 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_kernel(S *__restrict__ psi
     const u64 stride = (u64)1 << q;
     cd a[U], b[U];
     u64 i0[U];
+    if constexpr (sizeof(S) == 8) {
+        // complex64: raw loads without a condition (past the end: psi[0]), converted after the batch; a load
+        // inside the condition waited for its own conversion, one pair in flight per thread
+        S ra[U], rb[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 kk = base + (u64)u * SV_THREADS;
-        i0[u] = ins0(kk, q);
-        if (kk < npairs) {
-            a[u] = ldx<NT>(psi + i0[u]);
-            b[u] = ldx<NT>(psi + i0[u] + stride);
+        for (int u = 0; u < U; ++u) {
+            const u64 kk = base + (u64)u * SV_THREADS;
+            i0[u] = ins0(kk, q);
+            const u64 o = kk < npairs ? i0[u] : 0;
+            ra[u] = ldr<NT>(psi + o);
+            rb[u] = ldr<NT>(psi + (kk < npairs ? o + stride : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            hold(ra[u]);
+            hold(rb[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a[u] = to_cd(ra[u]);
+            b[u] = to_cd(rb[u]);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const u64 kk = base + (u64)u * SV_THREADS;
+            i0[u] = ins0(kk, q);
+            if (kk < npairs) {
+                a[u] = ldx<NT>(psi + i0[u]);
+                b[u] = ldx<NT>(psi + i0[u] + stride);
+            }
         }
     }
 #pragma unroll
@@ -186,10 +210,23 @@ __global__ __launch_bounds__(SV_THREADS) void sv_pair_lane_kernel(S *__restrict_
     const int mb = 1 << q;
     const bool lo = !(threadIdx.x & mb);   // blocks start on multiples of 256: bit q of x is bit q of the lane
     cd v[U];
+    if constexpr (sizeof(S) == 8) {   // complex64: raw loads, converted after the batch (as in sv_pair_kernel)
+        S r[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 x = base + (u64)u * SV_THREADS;
-        v[u] = x < dim ? ldx<1>(psi + x) : cmk(0.0, 0.0);
+        for (int u = 0; u < U; ++u) {
+            const u64 x = base + (u64)u * SV_THREADS;
+            r[u] = ldr<1>(psi + (x < dim ? x : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) hold(r[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = to_cd(r[u]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const u64 x = base + (u64)u * SV_THREADS;
+            v[u] = x < dim ? ldx<1>(psi + x) : cmk(0.0, 0.0);
+        }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -207,11 +244,25 @@ __global__ __launch_bounds__(SV_THREADS) void sv_one_kernel(S *__restrict__ psi,
     const u64 stride = (u64)1 << q;
     cd b[U];
     u64 i1[U];
+    if constexpr (sizeof(S) == 8) {   // complex64: raw loads (past the end: psi[0]), converted after the batch
+        S r[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 kk = base + (u64)u * SV_THREADS;
-        i1[u] = ins0(kk, q) + stride;
-        if (kk < npairs) b[u] = ldx<NT>(psi + i1[u]);
+        for (int u = 0; u < U; ++u) {
+            const u64 kk = base + (u64)u * SV_THREADS;
+            i1[u] = ins0(kk, q) + stride;
+            r[u] = ldr<NT>(psi + (kk < npairs ? i1[u] : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) hold(r[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = to_cd(r[u]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const u64 kk = base + (u64)u * SV_THREADS;
+            i1[u] = ins0(kk, q) + stride;
+            if (kk < npairs) b[u] = ldx<NT>(psi + i1[u]);
+        }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -235,14 +286,38 @@ __global__ __launch_bounds__(SV_THREADS) void sv_mono_kernel(S *__restrict__ psi
     // fixed-bound loops with guards keep v[][] in registers (no dynamic indexing)
     cd v[U][8];
     u64 base[U];
+    if constexpr (sizeof(S) == 8) {
+        // complex64: raw loads (past the end: psi[0]), converted after the batch; a load inside the
+        // per-lane condition waited for its own conversion, one amplitude in flight per thread
+        S r[U][8];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 b = b0 + (u64)u * SV_THREADS;
-        base[u] = ins0k(b, m.pos, m.k);
-        if (b < nbase) {
+        for (int u = 0; u < U; ++u) {
+            const u64 b = b0 + (u64)u * SV_THREADS;
+            base[u] = ins0k(b, m.pos, m.k);
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < m.nt) v[u][j] = ldx<NT>(psi + (base[u] | m.src[j]));
+            for (int j = 0; j < 8; ++j) {
+                r[u][j].x = r[u][j].y = 0.0f;
+                if (j < m.nt) r[u][j] = ldr<NT>(psi + (b < nbase ? (base[u] | m.src[j]) : 0));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) hold(r[u][j]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[u][j] = to_cd(r[u][j]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const u64 b = b0 + (u64)u * SV_THREADS;
+            base[u] = ins0k(b, m.pos, m.k);
+            if (b < nbase) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m.nt) v[u][j] = ldx<NT>(psi + (base[u] | m.src[j]));
+            }
         }
     }
 #pragma unroll
