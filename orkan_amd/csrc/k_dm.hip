@@ -1508,6 +1508,7 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
         }
     }
     cd f[E];
+    double2 pr[E], pc[E];   // exp_diag: the row's and the column's phase, loaded for every entry first
     bool chg[E], act[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -1521,10 +1522,11 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
             const int ar = gate_bits(r, gk, q0, q1, q2), ac = gate_bits(c, gk, q0, q1, q2);
             ok = ok && !((skipm >> (ar * 8 + ac)) & 1);
             if (ok) f[j] = a.f[ar * 8 + ac];
-        } else if (ok) {
-            // e^{-i(d_r - d_c)t} = (cr - i sr)(cc + i sc)   (src/circ/circ_packed.c:54-60)
-            const double2 pr = cs[r], pc = cs[c];
-            f[j] = cmk(pr.x * pc.x + pr.y * pc.y, -pr.y * pc.x + pr.x * pc.y);
+        } else {
+            // without a condition (an entry left alone reads cs[0]), all in flight before the first product:
+            // loaded and used inside the entry's condition, each pair was waited for in turn
+            pr[j] = cs[ok ? r : 0];
+            pc[j] = cs[ok ? c : 0];
         }
         chg[j] = ok;
         if constexpr (WL) {
@@ -1540,6 +1542,12 @@ __global__ __launch_bounds__(DM_THREADS) void dm_phase_kernel(S *__restrict__ rh
         raw[j].x = 0;
         raw[j].y = 0;
         if (act[j]) raw[j] = ldr<NT>(rho + x0 + (u64)j * DM_THREADS);
+    }
+    if constexpr (MODE != 0) {
+        // e^{-i(d_r - d_c)t} = (cr - i sr)(cc + i sc)   (src/circ/circ_packed.c:54-60)
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            if (chg[j]) f[j] = cmk(pr[j].x * pc[j].x + pr[j].y * pc[j].y, -pr[j].y * pc[j].x + pr[j].x * pc[j].y);
     }
     cd v[E];
 #pragma unroll
