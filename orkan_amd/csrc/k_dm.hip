@@ -1011,6 +1011,9 @@ __device__ __forceinline__ void perm_tile_entry(const PermTilePlan &p, u64 TR, u
 // used twice (stage p, write s; stage s, write p), the whole pair in one workgroup: split, one
 // workgroup would read entries another had rewritten.  (A second launch for the transposed pairs alone
 // spent most of its time dispatching the workgroups of the other tiles.)
+#ifndef ORK_PERMT_COLUMNS
+#define ORK_PERMT_COLUMNS 1
+#endif
 template <typename S, int LAYOUT, int NTA>
 __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict__ rho, PermTilePlan p) {
     __shared__ cd ts[W_MAX * LDS_STRIDE];
@@ -1023,20 +1026,31 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_tiles_kernel(S *__restrict
         const u64 w = g * (u64)p.tpw + (u64)tw;
         if (w >= p.ntiles) break;
         u64 TR, TC;
-        tri_decode(w, TR, TC);
+        // the packed layout walks down block columns (a block's column runs share their end lines with the
+        // blocks above and below); ownership follows the row-major index wr whatever the walk
+        if (LAYOUT == ORK_PACKED && ORK_PERMT_COLUMNS) {
+            u64 rr, cc;
+            tri_decode(p.ntiles - 1 - w, rr, cc);
+            const u64 nt = p.D >> p.L;
+            TR = nt - 1 - cc;
+            TC = nt - 1 - rr;
+        } else {
+            tri_decode(w, TR, TC);
+        }
+        const u64 wr = TR * (TR + 1) / 2 + TC;
         int sa, sb;
         const u64 PR = perm_src_tile(p, TR, sa), PC = perm_src_tile(p, TC, sb);
         const bool trans = PR < PC;
         const u64 pr = trans ? PC : PR, pc = trans ? PR : PC;
         const u64 wp = pr * (pr + 1) / 2 + pc;
-        if (wp < w) continue;   // the workgroup of p swaps both
+        if (wp < wr) continue;   // the workgroup of p swaps both
         const int phs = (p.ph[sa] - p.ph[sb]) & 3;
-        if (wp == w && !trans && phs == 0) continue;   // the tile keeps its value
+        if (wp == wr && !trans && phs == 0) continue;   // the tile keeps its value
         int pa, pb;
         (void)perm_src_tile(p, pr, pa);
         (void)perm_src_tile(p, pc, pb);
         const int php = (p.ph[pa] - p.ph[pb]) & 3;
-        const bool pair = wp != w;
+        const bool pair = wp != wr;
         cd vs[E], vp[E];
         S rs[E], rp[E];   // raw values: converted once every load of the tile is in flight
         u64 os[E], op[E];
