@@ -1366,7 +1366,16 @@ __global__ __launch_bounds__(DM_THREADS) void dm_perm_intile_kernel(S *__restric
     const u64 w = remap_group(blockIdx.x, p.ntiles, p.xcd);
     if (w >= p.ntiles) return;
     u64 TR, TC;
-    tri_decode(w, TR, TC);
+    if (LAYOUT == ORK_PACKED && ORK_PERMT_COLUMNS) {
+        // down block columns: a block's column runs share their end lines with the blocks above and below
+        u64 rr, cc;
+        tri_decode(p.ntiles - 1 - w, rr, cc);
+        const u64 nt = p.D >> 5;
+        TR = nt - 1 - cc;
+        TC = nt - 1 - rr;
+    } else {
+        tri_decode(w, TR, TC);
+    }
     const bool dg = TR == TC;
     cd v[4];
     S raw[4];
